@@ -1,0 +1,649 @@
+// C ABI of the solver: drop-in solves, batched plans, adjoint, preconditioner
+// and problem utilities.  Host code; launches the kernels of pcg_kernel.hip and
+// aux_kernels.hip through the helpers declared in internal.h.
+//
+// Reference mapping (see include/gbdpcg.h for the full table):
+//   host solve     <- solvePCG(h_S, h_Pinv, h_gamma, h_lambda, n, N, cfg)  include/interface.cuh:24-91
+//   device solve   <- solvePCG(n, N, d_S, d_Pinv, d_gamma, d_lambda, ...)  include/interface.cuh:94-146
+//   occupancy      <- checkPcgOccupancy                                    include/pcg.cuh:23-49
+// Differences kept on purpose: errors are returned, never exit(); the host solve
+// returns the iteration count (the reference returns 1, interface.cuh:90); n and
+// N are runtime values (the reference needs -DSTATE_SIZE/-DKNOT_POINTS,
+// include/interface.cuh:112); device buffers are cached, not allocated per call.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace gbdpcg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+}
+
+#define HIPCHK(expr)                                                            \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return GBDPCG_ERR_DEVICE;                                                 \
+    }                                                                           \
+  } while (0)
+
+constexpr uint32_t kMaxIterLimit = 100000;  // keeps a solve's epochs far from wrapping
+constexpr int kXcds = 8;                    // gfx950: 8 XCDs, workgroups dealt round-robin
+
+static size_t tsize(int dtype) { return dtype == GBDPCG_F64 ? 8 : 4; }
+static bool valid_dtype(int dtype) { return dtype == GBDPCG_F32 || dtype == GBDPCG_F64; }
+
+// ---------------------------------------------------------------------------
+// device properties
+// ---------------------------------------------------------------------------
+struct DeviceInfo {
+  int cus = 0;
+  size_t max_lds = 0;
+  bool ok = false;
+};
+
+static int device_info(int dev, DeviceInfo *out) {
+  static std::mutex mu;
+  static std::map<int, DeviceInfo> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) {
+    *out = it->second;
+    return GBDPCG_OK;
+  }
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  DeviceInfo di;
+  di.cus = prop.multiProcessorCount;
+  di.max_lds = prop.sharedMemPerBlock;
+  di.ok = true;
+  cache[dev] = di;
+  *out = di;
+  return GBDPCG_OK;
+}
+
+static int occupancy(int dtype, size_t lds, int *occ) {
+  static std::mutex mu;
+  static std::map<std::pair<int, size_t>, int> cache;
+  std::lock_guard<std::mutex> lk(mu);
+  auto key = std::make_pair(dtype, lds);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *occ = it->second;
+    return GBDPCG_OK;
+  }
+  HIPCHK(pcg_prepare(dtype, lds));
+  int o = 0;
+  HIPCHK(pcg_occupancy(dtype, lds, &o));
+  cache[key] = o;
+  *occ = o;
+  return GBDPCG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// per-(device, stream) workspace of the drop-in calls
+// ---------------------------------------------------------------------------
+struct SlotBuf {
+  uint64_t *p = nullptr;
+  size_t words = 0;
+  bool dirty = false;  // epochs inconsistent (after a timed-out wait): re-zero before use
+};
+
+struct Workspace {
+  std::mutex mu;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::map<uint64_t, SlotBuf> slots;  // keyed by layout signature
+  gbdpcg_result *d_result = nullptr;
+  int *d_err = nullptr;
+  int *d_perr = nullptr;
+  gbdpcg_result *h_result = nullptr;
+  int *h_err = nullptr;
+  void *d_pinv = nullptr;
+  size_t pinv_bytes = 0;
+  void *d_io = nullptr;
+  size_t io_bytes = 0;
+};
+
+static std::mutex g_ws_mu;
+static std::map<std::pair<int, hipStream_t>, std::unique_ptr<Workspace>> g_ws;
+
+static bool capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st != hipStreamCaptureStatusNone;
+}
+
+static int get_workspace(int dev, hipStream_t stream, Workspace **out) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto key = std::make_pair(dev, stream);
+  auto it = g_ws.find(key);
+  if (it != g_ws.end()) {
+    *out = it->second.get();
+    return GBDPCG_OK;
+  }
+  if (capturing(stream)) {
+    set_error("first solve on a stream must run outside graph capture (workspace allocation)");
+    return GBDPCG_ERR_INVALID;
+  }
+  auto ws = std::make_unique<Workspace>();
+  ws->device = dev;
+  ws->stream = stream;
+  HIPCHK(hipMalloc(&ws->d_result, sizeof(gbdpcg_result)));
+  HIPCHK(hipMalloc(&ws->d_err, 2 * sizeof(int)));
+  ws->d_perr = ws->d_err + 1;
+  HIPCHK(hipMemset(ws->d_err, 0, 2 * sizeof(int)));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ws->h_result), sizeof(gbdpcg_result), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ws->h_err), 2 * sizeof(int), hipHostMallocDefault));
+  *out = ws.get();
+  g_ws[key] = std::move(ws);
+  return GBDPCG_OK;
+}
+
+// Exchange-record buffer for one layout; zeroed when created or after a failure.
+static int get_slots(Workspace *ws, uint64_t sig, size_t words, uint64_t **out) {
+  auto &sb = ws->slots[sig];
+  if (sb.p == nullptr) {
+    if (capturing(ws->stream)) {
+      set_error("first solve of a new size must run outside graph capture (workspace allocation)");
+      return GBDPCG_ERR_INVALID;
+    }
+    HIPCHK(hipMalloc(&sb.p, words * sizeof(uint64_t)));
+    sb.words = words;
+    HIPCHK(hipMemsetAsync(sb.p, 0, words * sizeof(uint64_t), ws->stream));
+  } else if (sb.dirty) {
+    HIPCHK(fill_u64(sb.p, sb.words, 0, ws->stream));
+    sb.dirty = false;
+  }
+  *out = sb.p;
+  return GBDPCG_OK;
+}
+
+static int grow(void **p, size_t *cap, size_t need, hipStream_t s) {
+  if (*cap >= need) return GBDPCG_OK;
+  if (capturing(s)) {
+    set_error("buffer growth inside graph capture; run the same call once outside capture first");
+    return GBDPCG_ERR_INVALID;
+  }
+  if (*p) HIPCHK(hipFree(*p));
+  *p = nullptr;
+  *cap = 0;
+  HIPCHK(hipMalloc(p, need));
+  *cap = need;
+  return GBDPCG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// workgroup count for a single system
+// ---------------------------------------------------------------------------
+struct Layout {
+  uint32_t G = 0, stride = 1, grid = 0;
+  size_t lds = 0;
+  bool reg = false;  // register-resident kernel
+  int nt = 256;      // threads per workgroup
+};
+
+static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+// Latency-oriented choice: about 8 knots per workgroup (the per-iteration cost is
+// dominated by the two cross-workgroup exchanges, whose latency grows slowly with
+// the number of workgroups, while the on-chip matvec grows with the knots per
+// workgroup), all workgroups on one XCD when they fit there.
+static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t want_G, Layout *L) {
+  DeviceInfo di;
+  int rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  const uint32_t maxG = fused ? N / 2 : N;
+  if (maxG == 0) {
+    set_error("fused recurrence needs at least 2 knots per workgroup");
+    return GBDPCG_ERR_INVALID;
+  }
+  uint32_t G = want_G ? want_G : std::min<uint32_t>(std::max<uint32_t>(ceil_div(N, 8), 1), 32);
+  if (G > maxG) G = maxG;
+  while (item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype)) > di.max_lds && G < maxG && !want_G) ++G;
+  const size_t lds = item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype));
+  if (lds > di.max_lds) {
+    set_error("system n=%u N=%u with %u workgroups needs %zu B of LDS per workgroup (max %zu)", n, N, G, lds,
+              di.max_lds);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  if (G > (uint32_t)kMaxWgsPerSystem) {
+    set_error("too many workgroups per system (%u)", G);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  int occ = 0;
+  rc = occupancy(dtype, lds, &occ);
+  if (rc < 0) return rc;
+  const uint32_t per_xcd = (uint32_t)(di.cus / kXcds) * (uint32_t)occ;
+  const uint32_t total = (uint32_t)di.cus * (uint32_t)occ;
+  if (occ <= 0 || G > total) {
+    set_error("%u workgroups cannot be co-resident (capacity %u at %zu B LDS)", G, total, lds);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  L->G = G;
+  L->lds = lds;
+  L->stride = G <= per_xcd ? (uint32_t)kXcds : 1u;
+  L->grid = G * L->stride;
+  return GBDPCG_OK;
+}
+
+// Register-resident kernel (fused recurrence): reg_lpr(n) lanes per matrix row of
+// the owned knots plus one ghost knot each side.  Prefers 256 threads (one wave
+// per SIMD: every wave runs the uniform scalar work, so fewer waves cost less)
+// with up to 8 knots per workgroup; 512 threads when the knots need it.
+// Returns GBDPCG_ERR_UNSUPPORTED when the shape does not suit this kernel.
+static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, Layout *L) {
+  if (!reg_supported((int)n) || N < 4) return GBDPCG_ERR_UNSUPPORTED;
+  DeviceInfo di;
+  int rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  for (int nt : {256, 512}) {
+    // FP64 rows of n = 28 need ~176 VGPRs of matrix chunks: one wave per SIMD only
+    if (dtype == GBDPCG_F64 && n > 14 && nt > 256) continue;
+    const uint32_t kmax = reg_max_knots(n, nt);
+    if (kmax < 2) continue;
+    uint32_t G = want_G;
+    if (!G) {
+      const uint32_t target = std::min<uint32_t>(kmax, 8);
+      G = ceil_div(N, target);
+    }
+    if (G < 1 || N / G < 2 || ceil_div(N, G) > kmax || G > (uint32_t)kMaxWgsPerSystem) continue;
+    const size_t lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
+    if (lds > di.max_lds) continue;
+    HIPCHK(reg_prepare(dtype, (int)n, nt, lds));
+    int occ = 0;
+    HIPCHK(reg_occupancy(dtype, (int)n, nt, lds, &occ));
+    const uint32_t per_xcd = (uint32_t)(di.cus / kXcds) * (uint32_t)occ;
+    if (occ <= 0 || G > (uint32_t)di.cus * (uint32_t)occ) continue;
+    L->G = G;
+    L->lds = lds;
+    L->reg = true;
+    L->nt = nt;
+    L->stride = G <= per_xcd ? (uint32_t)kXcds : 1u;
+    L->grid = G * L->stride;
+    return GBDPCG_OK;
+  }
+  return GBDPCG_ERR_UNSUPPORTED;
+}
+
+static int validate_cfg(const gbdpcg_config *cfg) {
+  if (!cfg) {
+    set_error("null config");
+    return GBDPCG_ERR_INVALID;
+  }
+  if (cfg->max_iter > kMaxIterLimit) {
+    set_error("max_iter %u above the limit %u", cfg->max_iter, kMaxIterLimit);
+    return GBDPCG_ERR_INVALID;
+  }
+  if (!(cfg->exit_tol >= 0.0)) {
+    set_error("exit_tol must be >= 0");
+    return GBDPCG_ERR_INVALID;
+  }
+  if (cfg->exit_mode != GBDPCG_EXIT_ABS_ETA && cfg->exit_mode != GBDPCG_EXIT_REL_RESID) {
+    set_error("bad exit_mode %d", cfg->exit_mode);
+    return GBDPCG_ERR_INVALID;
+  }
+  if (cfg->recurrence != GBDPCG_RECUR_REFERENCE && cfg->recurrence != GBDPCG_RECUR_FUSED) {
+    set_error("bad recurrence %d", cfg->recurrence);
+    return GBDPCG_ERR_INVALID;
+  }
+  if (cfg->precond != GBDPCG_PRECOND_JACOBI && cfg->precond != GBDPCG_PRECOND_SS) {
+    set_error("bad precond %d", cfg->precond);
+    return GBDPCG_ERR_INVALID;
+  }
+  return GBDPCG_OK;
+}
+
+static int check_error_words(Workspace *ws, uint64_t sig) {
+  if (ws->h_err[0]) {
+    ws->slots[sig].dirty = true;
+    HIPCHK(hipMemsetAsync(ws->d_err, 0, sizeof(int), ws->stream));
+    HIPCHK(hipStreamSynchronize(ws->stream));
+    set_error("a cross-workgroup exchange wait timed out (workgroups not co-resident?)");
+    return GBDPCG_ERR_TIMEOUT;
+  }
+  if (ws->h_err[1]) {
+    HIPCHK(hipMemsetAsync(ws->d_perr, 0, sizeof(int), ws->stream));
+    HIPCHK(hipStreamSynchronize(ws->stream));
+    set_error("preconditioner: a diagonal block of S is singular");
+    return GBDPCG_ERR_INVALID;
+  }
+  return GBDPCG_OK;
+}
+
+// Core of every single-system device entry point.
+static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, const void *Pinv, const void *gamma,
+                             void *lambda, const void *glambda, void *ggamma, void *gS, uint32_t flags,
+                             const gbdpcg_config *cfg, bool sync, gbdpcg_result *h_out, gbdpcg_result *d_out) {
+  int rc = validate_cfg(cfg);
+  if (rc < 0) return rc;
+  if (!valid_dtype(dtype) || n == 0 || N == 0 || !S || !lambda) {
+    set_error("invalid argument (dtype/n/N/S/lambda)");
+    return GBDPCG_ERR_INVALID;
+  }
+  if ((flags & KF_FWD) && !gamma) {
+    set_error("gamma is null");
+    return GBDPCG_ERR_INVALID;
+  }
+  if ((flags & KF_ADJ) && (!glambda || !ggamma || !gS)) {
+    set_error("adjoint needs glambda, ggamma and gS");
+    return GBDPCG_ERR_INVALID;
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipStream_t stream = static_cast<hipStream_t>(cfg->stream);
+  Workspace *ws = nullptr;
+  rc = get_workspace(dev, stream, &ws);
+  if (rc < 0) return rc;
+  std::lock_guard<std::mutex> lk(ws->mu);
+
+  bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED;
+  if (fused && N < 2) fused = false;  // one knot: the reference recurrence
+  Layout L;
+  rc = GBDPCG_ERR_UNSUPPORTED;
+  if (fused && cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, cfg->grid[0], &L);
+  if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_REGISTER) {
+    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28}, N >= 4 and a fitting layout");
+    return GBDPCG_ERR_UNSUPPORTED;
+  }
+  if (rc == GBDPCG_ERR_UNSUPPORTED) rc = plan_single(dtype, n, N, fused, cfg->grid[0], &L);
+  if (rc < 0) return rc;
+
+  if (Pinv == nullptr) {
+    const size_t bytes = (size_t)3 * n * n * N * tsize(dtype);
+    rc = grow(&ws->d_pinv, &ws->pinv_bytes, bytes, stream);
+    if (rc < 0) return rc;
+    HIPCHK(precond_launch(dtype, n, N, S, ws->d_pinv, cfg->precond, ws->d_perr, stream));
+    Pinv = ws->d_pinv;
+  }
+  const size_t words = 2 * (size_t)L.G * rec_words(n, (uint32_t)(tsize(dtype) / 4));
+  const uint64_t sig = ((uint64_t)dtype) | ((uint64_t)n << 1) | ((uint64_t)L.G << 12) | ((uint64_t)N << 32);
+  uint64_t *slots = nullptr;
+  rc = get_slots(ws, sig, words, &slots);
+  if (rc < 0) return rc;
+
+  KSys one{};
+  one.S = S;
+  one.Pinv = Pinv;
+  one.gamma = gamma;
+  one.lambda = lambda;
+  one.glambda = glambda;
+  one.ggamma = ggamma;
+  one.gS = gS;
+  one.slots = slots;
+  one.result = d_out ? d_out : ws->d_result;
+  one.err = ws->d_err;
+  one.tol = cfg->exit_tol;
+  one.n = n;
+  one.N = N;
+  one.G = L.G;
+  one.max_iter = cfg->max_iter;
+  one.flags = flags | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u);
+  if (L.reg) {
+    HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
+  } else {
+    HIPCHK(pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
+  }
+  if (!sync) return GBDPCG_OK;
+
+  HIPCHK(hipMemcpyAsync(ws->h_result, one.result, sizeof(gbdpcg_result), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipMemcpyAsync(ws->h_err, ws->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  rc = check_error_words(ws, sig);
+  if (rc < 0) return rc;
+  if (h_out) *h_out = *ws->h_result;
+  if (ws->h_result->status == GBDPCG_TIMED_OUT || ws->h_result->adj_status == GBDPCG_TIMED_OUT) {
+    ws->slots[sig].dirty = true;
+    set_error("a cross-workgroup exchange wait timed out");
+    return GBDPCG_ERR_TIMEOUT;
+  }
+  return (flags & KF_FWD) ? ws->h_result->iters : ws->h_result->adj_iters;
+}
+
+template <typename T>
+static int solve_host_impl(int dtype, const T *h_S, const T *h_Pinv, const T *h_gamma, T *h_lambda, uint32_t n,
+                           uint32_t N, const gbdpcg_config *cfg) {
+  int rc = validate_cfg(cfg);
+  if (rc < 0) return rc;
+  if (!h_S || !h_gamma || !h_lambda || n == 0 || N == 0) {
+    set_error("invalid argument (null host pointer or zero size)");
+    return GBDPCG_ERR_INVALID;
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipStream_t stream = static_cast<hipStream_t>(cfg->stream);
+  Workspace *ws = nullptr;
+  rc = get_workspace(dev, stream, &ws);
+  if (rc < 0) return rc;
+  const size_t mat = (size_t)3 * n * n * N, vec = (size_t)n * N;
+  T *d_S, *d_P, *d_g, *d_l;
+  {
+    std::lock_guard<std::mutex> lk(ws->mu);
+    rc = grow(&ws->d_io, &ws->io_bytes, (2 * mat + 2 * vec) * sizeof(T), stream);
+    if (rc < 0) return rc;
+    d_S = static_cast<T *>(ws->d_io);
+    d_P = d_S + mat;
+    d_g = d_P + mat;
+    d_l = d_g + vec;
+    HIPCHK(hipMemcpyAsync(d_S, h_S, mat * sizeof(T), hipMemcpyHostToDevice, stream));
+    if (h_Pinv) HIPCHK(hipMemcpyAsync(d_P, h_Pinv, mat * sizeof(T), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_g, h_gamma, vec * sizeof(T), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_l, h_lambda, vec * sizeof(T), hipMemcpyHostToDevice, stream));
+  }
+  const int iters = solve_device_impl(dtype, n, N, d_S, h_Pinv ? d_P : nullptr, d_g, d_l, nullptr, nullptr, nullptr,
+                                      KF_FWD, cfg, true, nullptr, nullptr);
+  if (iters < 0) return iters;
+  HIPCHK(hipMemcpyAsync(h_lambda, d_l, vec * sizeof(T), hipMemcpyDeviceToHost, stream));
+  HIPCHK(hipStreamSynchronize(stream));
+  return iters;
+}
+
+template <typename T>
+static int solve_csr_impl(int dtype, const uint32_t *Sr, const uint32_t *Sc, const T *Sv, const uint32_t *Pr,
+                          const uint32_t *Pc, const T *Pv, const T *h_gamma, T *h_lambda, uint32_t n, uint32_t N,
+                          const gbdpcg_config *cfg);
+
+}  // namespace gbdpcg
+
+using namespace gbdpcg;
+
+// ---------------------------------------------------------------------------
+// exported entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *gbdpcg_version(void) { return "gbdpcg-mi355x 0.1.0"; }
+
+const char *gbdpcg_last_error(void) { return g_last_error.c_str(); }
+
+void gbdpcg_config_init(gbdpcg_config *cfg) {
+  if (!cfg) return;
+  memset(cfg, 0, sizeof *cfg);
+  cfg->exit_tol = 1e-6;  // include/constants.cuh:16
+  cfg->max_iter = 25;    // include/constants.cuh:15
+  cfg->grid[0] = 0;      // the reference ignores pcg_grid (interface.cuh:134); 0 = automatic here
+  cfg->grid[1] = cfg->grid[2] = 1;
+  cfg->block[0] = 64;  // include/constants.cuh:19 (the kernel uses its own 256)
+  cfg->block[1] = cfg->block[2] = 1;
+  cfg->empty_pinv = 1;  // include/types.cuh:33
+  cfg->exit_mode = GBDPCG_EXIT_ABS_ETA;
+  cfg->precond = GBDPCG_PRECOND_SS;
+  cfg->recurrence = GBDPCG_RECUR_REFERENCE;
+  cfg->stream = nullptr;
+}
+
+size_t gbdpcg_lds_bytes(int dtype, uint32_t n, uint32_t knots_per_wg, uint32_t wgs) {
+  if (!valid_dtype(dtype) || n == 0 || knots_per_wg == 0 || wgs == 0) return 0;
+  return item_lds_bytes(n, knots_per_wg, wgs, tsize(dtype));
+}
+
+int gbdpcg_check_occupancy(int dtype, uint32_t n, uint32_t N, uint32_t *max_wgs) {
+  if (!valid_dtype(dtype) || n == 0 || N == 0) {
+    set_error("invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  Layout L;
+  int rc = plan_single(dtype, n, N, false, 0, &L);
+  if (rc < 0) return rc;
+  int occ = 0;
+  rc = occupancy(dtype, L.lds, &occ);
+  if (rc < 0) return rc;
+  DeviceInfo di;
+  rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  if (max_wgs) *max_wgs = (uint32_t)(di.cus * occ);
+  return GBDPCG_OK;
+}
+
+int gbdpcg_solve_f32(const float *h_S, const float *h_Pinv, const float *h_gamma, float *h_lambda,
+                     uint32_t stateSize, uint32_t knotPoints, const gbdpcg_config *cfg) {
+  return solve_host_impl<float>(GBDPCG_F32, h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, cfg);
+}
+int gbdpcg_solve_f64(const double *h_S, const double *h_Pinv, const double *h_gamma, double *h_lambda,
+                     uint32_t stateSize, uint32_t knotPoints, const gbdpcg_config *cfg) {
+  return solve_host_impl<double>(GBDPCG_F64, h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, cfg);
+}
+
+int gbdpcg_solve_device_f32(uint32_t n, uint32_t N, const float *d_S, const float *d_Pinv, const float *d_gamma,
+                            float *d_lambda, const gbdpcg_config *cfg, gbdpcg_result *result) {
+  return solve_device_impl(GBDPCG_F32, n, N, d_S, d_Pinv, d_gamma, d_lambda, nullptr, nullptr, nullptr, KF_FWD, cfg,
+                           true, result, nullptr);
+}
+int gbdpcg_solve_device_f64(uint32_t n, uint32_t N, const double *d_S, const double *d_Pinv, const double *d_gamma,
+                            double *d_lambda, const gbdpcg_config *cfg, gbdpcg_result *result) {
+  return solve_device_impl(GBDPCG_F64, n, N, d_S, d_Pinv, d_gamma, d_lambda, nullptr, nullptr, nullptr, KF_FWD, cfg,
+                           true, result, nullptr);
+}
+int gbdpcg_solve_device_async_f32(uint32_t n, uint32_t N, const float *d_S, const float *d_Pinv,
+                                  const float *d_gamma, float *d_lambda, const gbdpcg_config *cfg,
+                                  gbdpcg_result *d_result) {
+  return solve_device_impl(GBDPCG_F32, n, N, d_S, d_Pinv, d_gamma, d_lambda, nullptr, nullptr, nullptr, KF_FWD, cfg,
+                           false, nullptr, d_result);
+}
+int gbdpcg_solve_device_async_f64(uint32_t n, uint32_t N, const double *d_S, const double *d_Pinv,
+                                  const double *d_gamma, double *d_lambda, const gbdpcg_config *cfg,
+                                  gbdpcg_result *d_result) {
+  return solve_device_impl(GBDPCG_F64, n, N, d_S, d_Pinv, d_gamma, d_lambda, nullptr, nullptr, nullptr, KF_FWD, cfg,
+                           false, nullptr, d_result);
+}
+
+int gbdpcg_adjoint_device_f32(uint32_t n, uint32_t N, const float *d_S, const float *d_Pinv, const float *d_gamma,
+                              float *d_lambda, const float *d_glambda, float *d_ggamma, float *d_gS, int forward,
+                              const gbdpcg_config *cfg, gbdpcg_result *result) {
+  return solve_device_impl(GBDPCG_F32, n, N, d_S, d_Pinv, d_gamma, d_lambda, d_glambda, d_ggamma, d_gS,
+                           (forward ? KF_FWD : 0u) | KF_ADJ, cfg, true, result, nullptr);
+}
+int gbdpcg_adjoint_device_f64(uint32_t n, uint32_t N, const double *d_S, const double *d_Pinv,
+                              const double *d_gamma, double *d_lambda, const double *d_glambda, double *d_ggamma,
+                              double *d_gS, int forward, const gbdpcg_config *cfg, gbdpcg_result *result) {
+  return solve_device_impl(GBDPCG_F64, n, N, d_S, d_Pinv, d_gamma, d_lambda, d_glambda, d_ggamma, d_gS,
+                           (forward ? KF_FWD : 0u) | KF_ADJ, cfg, true, result, nullptr);
+}
+
+int gbdpcg_solve_csr_f32(const uint32_t *Sr, const uint32_t *Sc, const float *Sv, const uint32_t *Pr,
+                         const uint32_t *Pc, const float *Pv, const float *h_gamma, float *h_lambda, uint32_t n,
+                         uint32_t N, const gbdpcg_config *cfg) {
+  return solve_csr_impl<float>(GBDPCG_F32, Sr, Sc, Sv, Pr, Pc, Pv, h_gamma, h_lambda, n, N, cfg);
+}
+int gbdpcg_solve_csr_f64(const uint32_t *Sr, const uint32_t *Sc, const double *Sv, const uint32_t *Pr,
+                         const uint32_t *Pc, const double *Pv, const double *h_gamma, double *h_lambda, uint32_t n,
+                         uint32_t N, const gbdpcg_config *cfg) {
+  return solve_csr_impl<double>(GBDPCG_F64, Sr, Sc, Sv, Pr, Pc, Pv, h_gamma, h_lambda, n, N, cfg);
+}
+
+int gbdpcg_precond_device(int dtype, uint32_t n, uint32_t N, const void *d_S, void *d_Pinv, int kind,
+                          void *stream) {
+  if (!valid_dtype(dtype) || n == 0 || N == 0 || !d_S || !d_Pinv ||
+      (kind != GBDPCG_PRECOND_JACOBI && kind != GBDPCG_PRECOND_SS)) {
+    set_error("gbdpcg_precond_device: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  DeviceInfo di;
+  int rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  if ((7 * (size_t)n * n + n) * tsize(dtype) > di.max_lds) {
+    set_error("gbdpcg_precond_device: n=%u too large for on-chip inversion", n);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  Workspace *ws = nullptr;
+  rc = get_workspace(dev, s, &ws);
+  if (rc < 0) return rc;
+  std::lock_guard<std::mutex> lk(ws->mu);
+  HIPCHK(precond_launch(dtype, n, N, d_S, d_Pinv, kind, ws->d_perr, s));
+  HIPCHK(hipMemcpyAsync(ws->h_err, ws->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (ws->h_err[1]) {
+    HIPCHK(hipMemsetAsync(ws->d_perr, 0, sizeof(int), s));
+    HIPCHK(hipStreamSynchronize(s));
+    set_error("gbdpcg_precond_device: a diagonal block of S is singular");
+    return GBDPCG_ERR_INVALID;
+  }
+  return GBDPCG_OK;
+}
+
+int gbdpcg_lqr_schur_device(int dtype, uint32_t n, uint32_t m, uint32_t N, const void *d_A, const void *d_B,
+                            const void *d_Qd, const void *d_Rd, void *d_S, void *stream) {
+  if (!valid_dtype(dtype) || n == 0 || N == 0 || !d_A || (m && (!d_B || !d_Rd)) || !d_Qd || !d_S) {
+    set_error("gbdpcg_lqr_schur_device: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  HIPCHK(schur_launch(dtype, n, m, N, d_A, d_B, d_Qd, d_Rd, d_S, static_cast<hipStream_t>(stream)));
+  return GBDPCG_OK;
+}
+
+int gbdpcg_warm_start_device(int dtype, uint32_t n, uint32_t N, void *d_lambda, void *d_gamma, const void *d_noise,
+                             void *stream) {
+  if (!valid_dtype(dtype) || n == 0 || N == 0 || !d_lambda || (d_noise && !d_gamma)) {
+    set_error("gbdpcg_warm_start_device: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  HIPCHK(warm_start_launch(dtype, n, N, d_lambda, d_gamma, d_noise, static_cast<hipStream_t>(stream)));
+  return GBDPCG_OK;
+}
+
+}  // extern "C"
+
+namespace gbdpcg {
+template <typename T>
+static int solve_csr_impl(int dtype, const uint32_t *Sr, const uint32_t *Sc, const T *Sv, const uint32_t *Pr,
+                          const uint32_t *Pc, const T *Pv, const T *h_gamma, T *h_lambda, uint32_t n, uint32_t N,
+                          const gbdpcg_config *cfg) {
+  if (!Sr || n == 0 || N == 0) {
+    set_error("CSR solve: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  const size_t mat = (size_t)3 * n * n * N;
+  std::vector<T> S(mat), P;
+  int rc = gbdpcg_csr_to_cbtd(dtype, Sr, Sc, Sv, n, N, S.data());
+  if (rc < 0) return rc;
+  if (Pr) {
+    P.resize(mat);
+    rc = gbdpcg_csr_to_cbtd(dtype, Pr, Pc, Pv, n, N, P.data());
+    if (rc < 0) return rc;
+  }
+  return solve_host_impl<T>(dtype, S.data(), Pr ? P.data() : nullptr, h_gamma, h_lambda, n, N, cfg);
+}
+}  // namespace gbdpcg

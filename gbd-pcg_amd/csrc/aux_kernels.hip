@@ -1,0 +1,295 @@
+// Auxiliary device kernels: fills, preconditioner formation from S, LQR Schur
+// complement formation, and the receding-horizon warm start.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "internal.h"
+
+namespace gbdpcg {
+
+__global__ void fill_u64_kernel(uint64_t *p, size_t count, uint64_t value) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+__global__ void fill_i32_kernel(int *p, size_t count, int value) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+
+hipError_t fill_u64(uint64_t *p, size_t count, uint64_t value, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, count, value);
+  return hipGetLastError();
+}
+
+hipError_t fill_i32(int *p, size_t count, int value, hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  size_t blocks = (count + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, count, value);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// preconditioner: block-Jacobi or symmetric stair, one workgroup per knot
+// ---------------------------------------------------------------------------
+
+constexpr int kAuxThreads = 256;
+
+// In-place Gauss-Jordan inversion with partial pivoting of the n x n row-major
+// matrix held in the left half of aug (n x 2n, right half set to I here).
+// Returns false (uniformly) if a pivot is exactly zero.
+template <typename T>
+__device__ bool gj_invert(T *aug, T *fac, int *piv, int n) {
+  const int n2 = 2 * n;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+    const int r = i / n, c = i - r * n;
+    aug[r * n2 + n + c] = r == c ? (T)1 : (T)0;
+  }
+  __syncthreads();
+  for (int col = 0; col < n; ++col) {
+    if (threadIdx.x == 0) {
+      int best = col;
+      T bv = __builtin_fabs((double)aug[col * n2 + col]);
+      for (int r = col + 1; r < n; ++r) {
+        const T v = __builtin_fabs((double)aug[r * n2 + col]);
+        if (v > bv) {
+          bv = v;
+          best = r;
+        }
+      }
+      piv[0] = best;
+      piv[1] = bv == (T)0 ? 1 : 0;
+    }
+    __syncthreads();
+    if (piv[1]) return false;
+    const int pr = piv[0];
+    if (pr != col) {
+      for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        const T t = aug[col * n2 + j];
+        aug[col * n2 + j] = aug[pr * n2 + j];
+        aug[pr * n2 + j] = t;
+      }
+    }
+    __syncthreads();
+    const T inv_p = (T)1 / aug[col * n2 + col];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) aug[col * n2 + j] *= inv_p;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) fac[r] = r == col ? (T)0 : aug[r * n2 + col];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n * n2; i += blockDim.x) {
+      const int r = i / n2, j = i - r * n2;
+      if (r != col) aug[i] -= fac[r] * aug[col * n2 + j];
+    }
+    __syncthreads();
+  }
+  return true;
+}
+
+// dst (n x n row-major) = theta_kk^-1, theta read column-major from S block (k,k).
+template <typename T>
+__device__ bool invert_diag(const T *S, int k, int n, T *aug, T *fac, int *piv, T *dst) {
+  const size_t bs = (size_t)3 * n * n;
+  const int n2 = 2 * n;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+    const int r = i / n, c = i - r * n;
+    aug[r * n2 + c] = S[(size_t)k * bs + (size_t)n * n + (size_t)c * n + r];
+  }
+  __syncthreads();
+  if (!gj_invert(aug, fac, piv, n)) return false;
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+    const int r = i / n, c = i - r * n;
+    dst[i] = aug[r * n2 + n + c];
+  }
+  __syncthreads();
+  return true;
+}
+
+// C = A * B, all n x n row-major in LDS.
+template <typename T>
+__device__ void mm(const T *A, const T *B, T *C, int n) {
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+    const int r = i / n, c = i - r * n;
+    T acc = 0;
+    for (int q = 0; q < n; ++q) acc += A[r * n + q] * B[q * n + c];
+    C[i] = acc;
+  }
+  __syncthreads();
+}
+
+// Pinv row k.  LDS: aug 2n^2 | inv_prev n^2 | inv_k n^2 | inv_next n^2 | t1 n^2 | t2 n^2 | fac n.
+template <typename T>
+__global__ void __launch_bounds__(kAuxThreads) precond_kernel(const T *__restrict__ S, T *__restrict__ P, int n,
+                                                             int N, int kind, int *err) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  T *aug = reinterpret_cast<T *>(lds_raw);
+  T *inv_prev = aug + 2 * n * n;
+  T *inv_k = inv_prev + n * n;
+  T *inv_next = inv_k + n * n;
+  T *t1 = inv_next + n * n;
+  T *t2 = t1 + n * n;
+  T *fac = t2 + n * n;
+  __shared__ int piv[2];
+  const int k = blockIdx.x;
+  const size_t bs = (size_t)3 * n * n;
+  T *row = P + (size_t)k * bs;
+  bool ok = invert_diag(S, k, n, aug, fac, piv, inv_k);
+  const bool ss = kind == GBDPCG_PRECOND_SS;
+  if (ok && ss && k > 0) ok = invert_diag(S, k - 1, n, aug, fac, piv, inv_prev);
+  if (ok && ss && k < N - 1) ok = invert_diag(S, k + 1, n, aug, fac, piv, inv_next);
+  if (!ok) {
+    if (threadIdx.x == 0) atomicOr(err, 1);
+    for (int i = threadIdx.x; i < (int)bs; i += blockDim.x) row[i] = (T)0;
+    return;
+  }
+  // diagonal block (column-major)
+  for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+    const int c = i / n, r = i - c * n;
+    row[(size_t)n * n + i] = inv_k[r * n + c];
+  }
+  // (k,k-1) = -inv_k phi_k inv_prev, phi_k = S(k,k-1)
+  if (ss && k > 0) {
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+      const int r = i / n, c = i - r * n;
+      t1[i] = S[(size_t)k * bs + (size_t)c * n + r];
+    }
+    __syncthreads();
+    mm(inv_k, t1, t2, n);
+    mm(t2, inv_prev, t1, n);
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+      const int c = i / n, r = i - c * n;
+      row[i] = -t1[r * n + c];
+    }
+  } else {
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) row[i] = (T)0;
+  }
+  __syncthreads();
+  // (k,k+1) = transpose of (k+1,k) = (-inv_next phi_{k+1} inv_k)^T
+  if (ss && k < N - 1) {
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+      const int r = i / n, c = i - r * n;
+      t1[i] = S[(size_t)(k + 1) * bs + (size_t)c * n + r];
+    }
+    __syncthreads();
+    mm(inv_next, t1, t2, n);
+    mm(t2, inv_k, t1, n);
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) {
+      const int c = i / n, r = i - c * n;
+      row[2 * (size_t)n * n + i] = -t1[c * n + r];
+    }
+  } else {
+    for (int i = threadIdx.x; i < n * n; i += blockDim.x) row[2 * (size_t)n * n + i] = (T)0;
+  }
+}
+
+hipError_t precond_launch(int dtype, uint32_t n, uint32_t N, const void *S, void *Pinv, int kind, int *err,
+                          hipStream_t stream) {
+  const size_t ts = dtype == GBDPCG_F64 ? 8 : 4;
+  const size_t lds = (7 * (size_t)n * n + n) * ts;
+  if (dtype == GBDPCG_F64) {
+    hipFuncSetAttribute(reinterpret_cast<const void *>(&precond_kernel<double>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(precond_kernel<double>, dim3(N), dim3(kAuxThreads), lds, stream,
+                       static_cast<const double *>(S), static_cast<double *>(Pinv), (int)n, (int)N, kind, err);
+  } else {
+    hipFuncSetAttribute(reinterpret_cast<const void *>(&precond_kernel<float>),
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(precond_kernel<float>, dim3(N), dim3(kAuxThreads), lds, stream,
+                       static_cast<const float *>(S), static_cast<float *>(Pinv), (int)n, (int)N, kind, err);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// LQR Schur complement: block row k of S from (A, B, Q, R), one workgroup per knot
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxThreads) schur_kernel(const T *__restrict__ A, const T *__restrict__ B,
+                                                           const T *__restrict__ Qd, const T *__restrict__ Rd,
+                                                           T *__restrict__ S, int n, int m, int N) {
+  const int k = blockIdx.x;
+  const size_t bs = (size_t)3 * n * n;
+  T *row = S + (size_t)k * bs;
+  for (int e = threadIdx.x; e < 3 * n * n; e += blockDim.x) {
+    const int col = e / n, r = e - col * n;
+    const int seg = col / n, c = col - seg * n;
+    T val = 0;
+    if (seg == 0) {
+      // phi_k = -A_{k-1} Q_{k-1}^-1
+      if (k > 0) val = -A[(size_t)(k - 1) * n * n + r * n + c] / Qd[(size_t)(k - 1) * n + c];
+    } else if (seg == 1) {
+      if (k > 0) {
+        const T *Ak = A + (size_t)(k - 1) * n * n;
+        const T *Bk = B + (size_t)(k - 1) * n * m;
+        const T *qk = Qd + (size_t)(k - 1) * n;
+        const T *rk = Rd + (size_t)(k - 1) * m;
+        for (int i = 0; i < n; ++i) val += Ak[r * n + i] * Ak[c * n + i] / qk[i];
+        for (int j = 0; j < m; ++j) val += Bk[r * m + j] * Bk[c * m + j] / rk[j];
+      }
+      if (r == c) val += (T)1 / Qd[(size_t)k * n + r];
+    } else {
+      // (k,k+1) = phi_{k+1}^T: element (r,c) = -A_k[c][r] / q_k[r]
+      if (k < N - 1) val = -A[(size_t)k * n * n + c * n + r] / Qd[(size_t)k * n + r];
+    }
+    row[e] = val;
+  }
+}
+
+hipError_t schur_launch(int dtype, uint32_t n, uint32_t m, uint32_t N, const void *A, const void *B,
+                        const void *Qd, const void *Rd, void *S, hipStream_t stream) {
+  if (dtype == GBDPCG_F64) {
+    hipLaunchKernelGGL(schur_kernel<double>, dim3(N), dim3(kAuxThreads), 0, stream, static_cast<const double *>(A),
+                       static_cast<const double *>(B), static_cast<const double *>(Qd),
+                       static_cast<const double *>(Rd), static_cast<double *>(S), (int)n, (int)m, (int)N);
+  } else {
+    hipLaunchKernelGGL(schur_kernel<float>, dim3(N), dim3(kAuxThreads), 0, stream, static_cast<const float *>(A),
+                       static_cast<const float *>(B), static_cast<const float *>(Qd),
+                       static_cast<const float *>(Rd), static_cast<float *>(S), (int)n, (int)m, (int)N);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// warm start: lambda[k] <- lambda[k+1] (k < N-1), gamma += noise
+// ---------------------------------------------------------------------------
+
+// One workgroup walks the vector in chunks: a chunk is read completely before
+// any of it is written, and a later chunk only reads positions no earlier chunk
+// has written (the shift moves values towards lower indices), so the in-place
+// shift is race free.
+template <typename T>
+__global__ void __launch_bounds__(kAuxThreads) warm_start_kernel(T *lambda, T *gamma, const T *noise, int n, int N) {
+  const int total = n * N;
+  const int limit = n * (N - 1);
+  for (int base = 0; base < limit; base += kAuxThreads) {
+    const int i = base + threadIdx.x;
+    T v = 0;
+    if (i < limit) v = lambda[i + n];
+    __syncthreads();
+    if (i < limit) lambda[i] = v;
+    __syncthreads();
+  }
+  if (noise != nullptr) {
+    for (int i = threadIdx.x; i < total; i += kAuxThreads) gamma[i] += noise[i];
+  }
+}
+
+hipError_t warm_start_launch(int dtype, uint32_t n, uint32_t N, void *lambda, void *gamma, const void *noise,
+                             hipStream_t stream) {
+  if (dtype == GBDPCG_F64) {
+    hipLaunchKernelGGL(warm_start_kernel<double>, dim3(1), dim3(kAuxThreads), 0, stream,
+                       static_cast<double *>(lambda), static_cast<double *>(gamma),
+                       static_cast<const double *>(noise), (int)n, (int)N);
+  } else {
+    hipLaunchKernelGGL(warm_start_kernel<float>, dim3(1), dim3(kAuxThreads), 0, stream,
+                       static_cast<float *>(lambda), static_cast<float *>(gamma), static_cast<const float *>(noise),
+                       (int)n, (int)N);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace gbdpcg

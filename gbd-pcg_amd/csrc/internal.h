@@ -1,0 +1,123 @@
+// Internal definitions shared by the device kernels (kernels.hip) and the host
+// runtime (api.cpp).  Not part of the public ABI.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/gbdpcg.h"
+
+namespace gbdpcg {
+
+// Threads per workgroup of the PCG kernel: four wave64s, one per SIMD.
+constexpr int kThreads = 256;
+// Column chunks per matrix row handled by one lane group (power of two).
+constexpr int kColSplit = 4;
+// Upper bound of workgroups cooperating on one system (limits the LDS scratch
+// that gathers the per-workgroup partial sums).
+constexpr int kMaxWgsPerSystem = 1024;
+// Polls of one exchange word before a wait is declared failed (about a second).
+constexpr uint32_t kSpinLimit = 1u << 21;
+
+// Flags of one system descriptor.
+enum : uint32_t {
+  KF_FWD = 1u,    // forward solve S lambda = gamma
+  KF_ADJ = 2u,    // adjoint solve S g_gamma = g_lambda (+ g_S)
+  KF_REL = 4u,    // relative-residual exit test instead of |r'z| < tol
+  KF_FUSED = 8u,  // single-reduction (Chronopoulos-Gear) recurrence
+};
+
+// Device-side descriptor of one system (one PCG instance).
+struct KSys {
+  const void *S;
+  const void *Pinv;
+  const void *gamma;
+  void *lambda;
+  const void *glambda;
+  void *ggamma;
+  void *gS;
+  uint64_t *slots;        // exchange words of this system: 2 * G * rec_words
+  gbdpcg_result *result;  // result record (may be null)
+  int *err;               // launch-wide error word: bit 0 set on a timed-out wait
+  double tol;
+  uint32_t n, N, G, max_iter;
+  uint32_t flags;
+  uint32_t pad_;
+};
+
+__host__ __device__ constexpr uint32_t padded_n(uint32_t n) { return (n + 3u) & ~3u; }
+
+// Columns handled by one lane of a kColSplit group (multiple of 4), and the padded
+// row stride of the on-chip row-major block rows.
+__host__ __device__ constexpr uint32_t col_chunk(uint32_t n) {
+  const uint32_t cols = 3u * padded_n(n);
+  return ((cols + 4u * kColSplit - 1u) / (4u * kColSplit)) * 4u;
+}
+__host__ __device__ constexpr uint32_t row_stride(uint32_t n) { return col_chunk(n) * kColSplit; }
+
+// Register kernel: lanes sharing one matrix row, and the columns each holds.
+__host__ __device__ constexpr uint32_t reg_lpr(uint32_t n) { return n <= 2 ? 1u : 2u; }
+__host__ __device__ constexpr uint32_t reg_chunk(uint32_t n) {
+  return 4u * ((3u * padded_n(n) + 4u * reg_lpr(n) - 1u) / (4u * reg_lpr(n)));
+}
+
+// Words (64-bit) of one exchange record: kRecScalars scalar slots (3 per wave of
+// the publishing workgroup, up to 8 waves) + left edge (2 knots of n) + right
+// edge (2 knots of n), each value split into 32-bit payload words.
+constexpr uint32_t kRecScalars = 24;
+__host__ __device__ constexpr uint32_t rec_words(uint32_t n, uint32_t word_per_value) {
+  return (kRecScalars + 4u * n) * word_per_value;
+}
+
+// Number of T elements of the dynamic LDS used by one workgroup owning K knots.
+// Layout (T elements): S rows K*n*RS | Pinv rows (K+2)*n*RS (one ghost knot row
+// each side) | 7 ghost-extended vectors of VL = (K+4)*nP + RS (two ghost knots
+// each side, plus the read-ahead of the last chunk) | 12 wave partials |
+// 3*G gathered partials | 16 control words.
+__host__ __device__ inline size_t item_lds_elems(uint32_t n, uint32_t K, uint32_t G) {
+  const size_t nP = padded_n(n), RS = row_stride(n);
+  const size_t VL = (size_t)(K + 4) * nP + RS;
+  return (2 * (size_t)K + 2) * n * RS + 7 * VL + 12 + 3 * (size_t)G + 16;
+}
+
+inline size_t item_lds_bytes(uint32_t n, uint32_t K, uint32_t G, size_t tsize) {
+  return item_lds_elems(n, K, G) * tsize;
+}
+
+// Knot range owned by rank g of G over N knots: [first, last).
+__host__ __device__ inline uint32_t knot_begin(uint32_t g, uint32_t G, uint32_t N) {
+  return (uint32_t)(((uint64_t)g * N) / G);
+}
+
+// Item encoding in the batch schedule: system index (20 bits) | rank (12 bits).
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+__host__ __device__ inline uint32_t make_item(uint32_t sys, uint32_t rank) {
+  return (sys << 12) | rank;
+}
+
+}  // namespace gbdpcg
+
+// ---- host-side helpers implemented next to the kernels ----
+#include <hip/hip_runtime_api.h>
+namespace gbdpcg {
+hipError_t pcg_prepare(int dtype, size_t lds_bytes);
+hipError_t pcg_occupancy(int dtype, size_t lds_bytes, int *wgs_per_cu);
+hipError_t pcg_launch(int dtype, const KSys &one, const KSys *systems, const uint32_t *items, uint32_t rounds,
+                      uint32_t stride, uint32_t grid, size_t lds_bytes, hipStream_t stream);
+// aux_kernels.hip
+hipError_t fill_u64(uint64_t *p, size_t count, uint64_t value, hipStream_t stream);
+hipError_t fill_i32(int *p, size_t count, int value, hipStream_t stream);
+hipError_t precond_launch(int dtype, uint32_t n, uint32_t N, const void *S, void *Pinv, int kind, int *err,
+                          hipStream_t stream);
+hipError_t schur_launch(int dtype, uint32_t n, uint32_t m, uint32_t N, const void *A, const void *B,
+                        const void *Qd, const void *Rd, void *S, hipStream_t stream);
+hipError_t warm_start_launch(int dtype, uint32_t n, uint32_t N, void *lambda, void *gamma, const void *noise,
+                             hipStream_t stream);
+// pcg_reg_kernel.hip (register-resident latency kernel, fused recurrence)
+bool reg_supported(int n);
+size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt);
+uint32_t reg_max_knots(uint32_t n, int nt);
+hipError_t reg_prepare(int dtype, int n, int nt, size_t lds);
+hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu);
+hipError_t reg_launch(int dtype, int n, int nt, const KSys &one, const KSys *systems, const uint32_t *items,
+                      uint32_t rounds, uint32_t stride, uint32_t grid, size_t lds, hipStream_t stream);
+}  // namespace gbdpcg

@@ -1,0 +1,155 @@
+"""GPU parity of the single-system solve (C ABI through ctypes) against the CPU oracle.
+
+Bar (per system): iteration count within +-1 of the oracle at the same precision;
+lambda within 1e-4 (FP32) / 1e-10 (FP64) relative error of the FP64 oracle run for
+the same number of iterations; the error against the FP64 direct solve is reported.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+LAM_TOL = {np.float32: 1e-4, np.float64: 1e-10}
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def lqr_system(n, N, seed, dtype, precond=2):
+    import gbdpcg
+    S, g = gbdpcg.lqr_generate(n, n // 2 if n > 1 else 1, N, seed, dtype=np.float64)
+    P = O.precond(S, n, N, precond)
+    return S.astype(dtype), P.astype(dtype), g.astype(dtype)
+
+
+def gpu_solve(S, P, g, lam0, n, N, dtype, **kw):
+    torch = _torch()
+    import gbdpcg
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    dS = torch.from_numpy(np.ascontiguousarray(S)).to("cuda", tdt)
+    dP = torch.from_numpy(np.ascontiguousarray(P)).to("cuda", tdt) if P is not None else None
+    dg = torch.from_numpy(np.ascontiguousarray(g)).to("cuda", tdt)
+    dl = torch.from_numpy(np.ascontiguousarray(lam0)).to("cuda", tdt)
+    res = gbdpcg.solve(dS, dg, dl, n, N, dP, **kw)
+    return dl.cpu().numpy(), res
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def check_against_oracle(S, P, g, n, N, dtype, tol, max_iter, mode, recurrence, wgs=0, lam0=None):
+    import gbdpcg
+    lam0 = np.zeros(n * N, dtype) if lam0 is None else lam0.astype(dtype)
+    lam_gpu, res = gpu_solve(S, P, g, lam0, n, N, dtype, tol=tol, max_iter=max_iter, exit_mode=mode,
+                             recurrence=recurrence, wgs=wgs)
+    fused = recurrence == gbdpcg.RECUR_FUSED
+    # same-precision oracle: iteration count
+    _, it_same, st_same, *_ = O.pcg(S, P, g, lam0, n, N, tol, max_iter, mode, dtype, fused=fused)
+    assert abs(res.iters - it_same) <= 1, (res.iters, it_same)
+    # FP64 oracle for the same number of iterations: lambda
+    lam_ref, it_ref, *_ = O.pcg(S.astype(np.float64), P.astype(np.float64), g.astype(np.float64),
+                                lam0.astype(np.float64), n, N, 0.0, res.iters, mode, np.float64)
+    assert it_ref == res.iters
+    err = rel(lam_gpu, lam_ref)
+    assert err <= LAM_TOL[dtype], (err, res.iters)
+    return res, lam_gpu, it_same
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_readme_example(gpu, dtype):
+    d = json.load(open(os.path.join(HERE, "golden", "readme_example.json")))
+    n, N = d["n"], d["N"]
+    S = np.array(d["S"], dtype)
+    P = np.where(S != 0, 1 / np.where(S != 0, S, 1), 0).astype(dtype)
+    g = np.array(d["gamma"], dtype)
+    exact = O.dense_solve(S.astype(np.float64), g.astype(np.float64), n, N)
+    if dtype == np.float64:
+        res, lam, it_same = check_against_oracle(S, P, g, n, N, dtype, d["tol"], d["max_iter"], 0, 0)
+        assert res.iters == 6 and it_same == 6
+        assert rel(lam, exact) < 1e-10
+    else:
+        # S has cond ~1.6e3 and the example's Pinv is indefinite: in FP32 the
+        # iteration count moves between 9 and 12 with the summation order alone
+        # (the oracle itself gives 10 or 12 depending on how Pinv is rounded), so
+        # this case is held to convergence and to lambda, not to +-1 iterations.
+        lam, res = gpu_solve(S, P, g, np.zeros(n * N, dtype), n, N, dtype, tol=d["tol"], max_iter=d["max_iter"])
+        _, it_same, *_ = O.pcg(S, P, g, np.zeros(n * N, dtype), n, N, d["tol"], d["max_iter"], 0, dtype)
+        assert res.iters <= it_same + 3
+        assert rel(lam, exact) < 1e-4
+    print(f"readme {dtype.__name__}: iters gpu={res.iters} oracle={it_same} err_vs_closed={rel(lam, exact):.2e}")
+    assert res.status == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("N", [32, 64, 128, 256])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_lqr_n14(gpu, dtype, N, recurrence):
+    n = 14
+    S, P, g = lqr_system(n, N, 2000 + N, dtype)
+    res, lam, it_same = check_against_oracle(S, P, g, n, N, dtype, 1e-5, 173, 1, recurrence)
+    exact = O.block_solve(S.astype(np.float64), g.astype(np.float64), n, N)
+    print(f"n14 N={N} {dtype.__name__} rec={recurrence}: iters gpu={res.iters} oracle={it_same} "
+          f"err_vs_direct={rel(lam, exact):.2e}")
+    assert res.status == 0
+
+
+@pytest.mark.parametrize("wgs", [1, 2, 3, 5, 16, 40])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_workgroup_splits(gpu, wgs, recurrence):
+    n, N, dtype = 6, 40, np.float64
+    if recurrence == 1 and wgs > N // 2:
+        pytest.skip("fused form needs 2 knots per workgroup")
+    S, P, g = lqr_system(n, N, 77, dtype)
+    check_against_oracle(S, P, g, n, N, dtype, 1e-8, 173, 1, recurrence, wgs=wgs)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 12, 28])
+def test_state_sizes(gpu, n):
+    N, dtype = 9, np.float64
+    S, P, g = lqr_system(n, N, 5 + n, dtype)
+    check_against_oracle(S, P, g, n, N, dtype, 1e-9, 200, 1, 0)
+
+
+def test_edge_cases(gpu):
+    import gbdpcg
+    n, N, dtype = 4, 5, np.float64
+    S, P, g = lqr_system(n, N, 11, dtype)
+    # max_iter = 0: lambda unchanged, status MAX_ITER
+    lam0 = np.linspace(-1, 1, n * N)
+    lam, res = gpu_solve(S, P, g, lam0, n, N, dtype, tol=1e-9, max_iter=0)
+    assert res.iters == 0 and res.status == gbdpcg.MAX_ITER and np.array_equal(lam, lam0)
+    # zero right-hand side and guess: converged at once
+    lam, res = gpu_solve(S, P, np.zeros(n * N), np.zeros(n * N), n, N, dtype, tol=1e-9, max_iter=10)
+    assert res.iters == 0 and res.status == gbdpcg.CONVERGED and not lam.any()
+    # huge tolerance: exactly one iteration (the reference counts iter++ before break)
+    lam, res = gpu_solve(S, P, g, np.zeros(n * N), n, N, dtype, tol=1e30, max_iter=10)
+    assert res.iters == 1
+    # one knot
+    S1, P1, g1 = lqr_system(n, 1, 3, dtype)
+    check_against_oracle(S1, P1, g1, n, 1, dtype, 1e-12, 50, 1, 0)
+    # two knots
+    S2, P2, g2 = lqr_system(n, 2, 3, dtype)
+    check_against_oracle(S2, P2, g2, n, 2, dtype, 1e-12, 50, 1, 1)
+
+
+def test_warm_start_and_repeat(gpu):
+    """Repeated calls on one stream reuse the exchange records (epochs continue)."""
+    n, N, dtype = 14, 64, np.float32
+    S, P, g = lqr_system(n, N, 99, dtype)
+    first = None
+    for _ in range(5):
+        lam, res = gpu_solve(S, P, g, np.zeros(n * N, dtype), n, N, dtype, tol=1e-5, max_iter=173, exit_mode=1)
+        if first is None:
+            first = (lam, res.iters)
+        assert res.iters == first[1] and np.array_equal(lam, first[0])  # deterministic
+    # warm start from the solution: converges in at most a couple of iterations
+    lam2, res2 = gpu_solve(S, P, g, first[0], n, N, dtype, tol=1e-5, max_iter=173, exit_mode=1)
+    assert res2.iters <= 2
