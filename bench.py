@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the block-tridiagonal PCG solve on one MI355X.
+
+Headline (value): microseconds per plain single solve of the LQR Schur system
+n=14 (m=7), N=128, FP32, symmetric-stair preconditioner, exit at relative
+residual 1e-5 (max_iter 173), lambda0 = 0, inputs resident in HBM, one solve per
+step launched back to back through the drop-in device entry point.
+
+Also reported on the same line:
+  * roofline: HBM bytes of the solve's algorithmic traffic (S and Pinv read once,
+    gamma read, lambda written) / the solve kernel's average duration (HIP events
+    around each launch on the launch stream);
+  * batched: forward + adjoint solves per second for 64 systems of the same shape
+    in one persistent launch (graph-replayed), and its ratio to back-to-back
+    plain solves;
+  * cpu_baseline: the FP64 host restatement of the reference iteration (oracle/,
+    single thread) on a time-bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "gbd-pcg_amd"))
+
+METRIC = "µs/solve to 1e-5 rel. resid (n=14,N=128); batched fwd+adjoint solves/sec"
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-batched", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the solver runs each system on one "
+              "device; see DESIGN.md for how the batched path would shard across GPUs.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import gbdpcg
+
+    n, m, N, tol, max_iter = 14, 7, 128, 1e-5, 173
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+
+    # synthetic LQR Schur system (seeded), SS preconditioner formed on the device
+    S_h, g_h = gbdpcg.lqr_generate(n, m, N, 2128, dtype=np.float32)
+    S = torch.from_numpy(S_h).to(dev)
+    g = torch.from_numpy(g_h).to(dev)
+    P = torch.empty_like(S)
+    gbdpcg.precond_device(S, P, n, N, gbdpcg.PRECOND_SS)
+    kw = dict(tol=tol, max_iter=max_iter, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+
+    # one zeroed lambda per step so every step is a full solve from lambda0 = 0
+    K, W = args.steps, args.warmup
+    lams = torch.zeros((K + W, n * N), device=dev)
+    dres = torch.zeros(gbdpcg.RESULT_BYTES, dtype=torch.uint8, device=dev)
+    res = gbdpcg.solve(S, g, torch.zeros(n * N, device=dev), n, N, P, **kw)
+    iters = res.iters
+    torch.cuda.synchronize()
+    for i in range(W):
+        gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * K)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for i in range(K):
+        ev[2 * i].record(stream)
+        gbdpcg.solve_async(S, g, lams[W + i], n, N, P, dres, **kw)
+        ev[2 * i + 1].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_start.elapsed_time(t_end)
+    us_per_solve = total_ms * 1000.0 / K
+    kern_us = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(K)])) * 1000.0
+    lam_last = lams[W + K - 1].cpu().numpy()
+
+    # end-to-end drop-in call (launch + wait + iteration count back to the host)
+    lam_tmp = torch.zeros(n * N, device=dev)
+    t0 = time.perf_counter()
+    reps = 100
+    for _ in range(reps):
+        lam_tmp.zero_()
+        gbdpcg.solve(S, g, lam_tmp, n, N, P, **kw)
+    sync_us = (time.perf_counter() - t0) * 1e6 / reps
+    # host-pointer drop-in (PCIe transfers included)
+    P_h = P.cpu().numpy()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        lh = np.zeros(n * N, dtype=np.float32)
+        gbdpcg.solve_host(S_h, g_h, lh, n, N, P_h, **kw)
+    host_us = (time.perf_counter() - t0) * 1e6 / 20
+    # the reference recurrence (two reductions per iteration) through the same entry point
+    kw_ref = dict(kw, recurrence=gbdpcg.RECUR_REFERENCE)
+    nref = min(K, 200)
+    lams.zero_()
+    for i in range(W):
+        gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw_ref)
+    torch.cuda.synchronize()
+    t_start.record(stream)
+    for i in range(nref):
+        gbdpcg.solve_async(S, g, lams[W + i], n, N, P, dres, **kw_ref)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    ref_us = t_start.elapsed_time(t_end) * 1000.0 / nref
+    lams[:W + nref].zero_()
+
+    # roofline of the solve kernel: algorithmic HBM bytes per launch
+    ts = 4
+    hbm_bytes = (2 * 3 * n * n * N + 2 * n * N) * ts
+    achieved = hbm_bytes / (kern_us * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "note": "latency-bound: one cross-workgroup exchange per iteration; see per_iteration_us"}
+
+    # batched forward + adjoint (64 systems, same shape), one persistent launch
+    batched = None
+    if not args.no_batched:
+        B = 64
+        systems = []
+        for b in range(B):
+            Sb_h, gb_h = gbdpcg.lqr_generate(n, m, N, 4000 + b, dtype=np.float32)
+            Sb = torch.from_numpy(Sb_h).to(dev)
+            Pb = torch.empty_like(Sb)
+            gbdpcg.precond_device(Sb, Pb, n, N, gbdpcg.PRECOND_SS)
+            systems.append(dict(S=Sb, Pinv=Pb, gamma=torch.from_numpy(gb_h).to(dev),
+                                lam=torch.zeros(n * N, device=dev), n=n, N=N, tol=tol, max_iter=max_iter,
+                                glam=torch.from_numpy(gbdpcg.randn(n * N, 7000 + b)).to(dev),
+                                ggamma=torch.zeros(n * N, device=dev), gS=torch.zeros_like(Sb), adjoint=True))
+        plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+        plan.run()
+        bres = plan.results()
+        graph = torch.cuda.CUDAGraph()
+        gs = torch.cuda.Stream()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=gs):
+            for d in systems:
+                d["lam"].zero_()
+            plan.run(stream=gs)
+        for _ in range(3):
+            graph.replay()
+        torch.cuda.synchronize()
+        nb = 30
+        t_start.record(stream)
+        for _ in range(nb):
+            graph.replay()
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        batch_us = t_start.elapsed_time(t_end) * 1000.0 / nb
+        rate = B / (batch_us * 1e-6)
+        batched = {"systems": B, "us_per_batch": round(batch_us, 2), "fwd_adj_solves_per_s": round(rate, 1),
+                   "vs_back_to_back_plain": round(rate / (1e6 / us_per_solve), 2),
+                   "mean_iters_fwd": float(np.mean([r.iters for r in bres])),
+                   "mean_iters_adj": float(np.mean([r.adj_iters for r in bres])),
+                   "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
+                   "plan": plan.info()}
+
+    # CPU baseline: FP64 host restatement of the reference iteration (oracle/), one thread
+    from oracle import oracle as O
+    S64, g64, P64 = S_h.astype(np.float64), g_h.astype(np.float64), P_h.astype(np.float64)
+    lam0 = np.zeros(n * N)
+    O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
+    cnt, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        _, it_cpu, *_ = O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
+        cnt += 1
+    cpu_us = (time.perf_counter() - t0) * 1e6 / cnt
+    cpu = {"value": round(cpu_us, 2), "unit": "us/solve", "cores": 1, "kind": "port",
+           "sample": f"{cnt} FP64 solves of the same n=14,N=128 system (oracle/pcg_oracle.c, {it_cpu} iterations each)"}
+
+    # accuracy of the last timed solve against the FP64 oracle at the same iteration count
+    lam_ref, *_ = O.pcg(S64, P64, g64, lam0, n, N, 0.0, iters)
+    err = float(np.linalg.norm(lam_last - lam_ref) / np.linalg.norm(lam_ref))
+
+    out = {
+        "metric": METRIC, "value": round(us_per_solve, 3), "unit": "us/solve", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(us_per_solve / 1000.0, 6), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded LQR Schur complement, on-device SS Pinv)",
+        "config": {"workload": "plain PCG single solve, n=14 m=7 N=128, symmetric-stair Pinv, rel resid 1e-5, "
+                               "max_iter 173, lambda0=0", "n": n, "N": N, "recurrence": "fused (Chronopoulos-Gear)",
+                   "iters": iters, "rel_err_vs_fp64_oracle": err},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
+        "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
+        "reference_recurrence_us": round(ref_us, 2), "batched": batched,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

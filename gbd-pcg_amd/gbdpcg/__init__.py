@@ -146,6 +146,8 @@ def version() -> str:
 
 def _dtype_code(dt) -> int:
     s = str(dt)
+    if not s.startswith("torch."):
+        s = str(np.dtype(dt))
     if s in ("torch.float64", "float64"):
         return F64
     if s in ("torch.float32", "float32"):

@@ -1,0 +1,140 @@
+"""GPU adjoint (differentiation) path against the oracle, FP64 finite differences
+and the dense FP64 gradient.
+
+Bars: g_gamma within 1e-4 (FP32) / 1e-10 (FP64) of the FP64 oracle PCG run for the
+same number of adjoint iterations; g_S equal (to rounding) to the formula applied
+to the returned lambda and g_gamma; with a tight FP64 tolerance, g_S and g_gamma
+within 1e-8 of the dense FP64 gradient and directional derivatives within 1e-6 of
+central finite differences (eps = 1e-6) on 32 symmetric block-tridiagonal
+directions.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", torch.float64 if dtype == np.float64 else torch.float32)
+
+
+def system(n, N, seed, dtype):
+    import gbdpcg
+    S, g = gbdpcg.lqr_generate(n, n // 2, N, seed, dtype=np.float64)
+    P = O.precond(S, n, N, 2)
+    glam = gbdpcg.randn(n * N, seed + 99, dtype=np.float64)
+    return S, P, g, glam
+
+
+def run_adjoint(S, P, g, glam, n, N, dtype, forward=True, lam_in=None, **kw):
+    import gbdpcg
+    dS, dP, dg, dgl = _t(S.astype(dtype), dtype), _t(P.astype(dtype), dtype), _t(g.astype(dtype), dtype), \
+        _t(glam.astype(dtype), dtype)
+    lam0 = np.zeros(n * N) if lam_in is None else lam_in
+    dl = _t(lam0.astype(dtype), dtype)
+    dgg = _t(np.zeros(n * N, dtype), dtype)
+    dgS = _t(np.full(3 * n * n * N, np.nan, dtype), dtype)
+    res = gbdpcg.adjoint(dS, dg, dl, dgl, dgg, dgS, n, N, dP, forward=forward, **kw)
+    return dl.cpu().numpy().astype(np.float64), dgg.cpu().numpy().astype(np.float64), \
+        dgS.cpu().numpy().astype(np.float64), res
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+@pytest.mark.parametrize("N", [32, 128])
+def test_adjoint_against_oracle(gpu, dtype, recurrence, N):
+    n = 14
+    S, P, g, glam = system(n, N, 4000 + N, dtype)
+    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=1e-5, max_iter=173, exit_mode=1,
+                                   recurrence=recurrence)
+    assert res.status == 0 and res.adj_status == 0
+    tol = 1e-4 if dtype == np.float32 else 1e-10
+    lam_ref, it, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 0.0, res.iters)
+    assert rel(lam, lam_ref) <= tol
+    gg_ref, it2, *_ = O.pcg(S, P, glam, np.zeros(n * N), n, N, 0.0, res.adj_iters)
+    assert rel(gg, gg_ref) <= tol
+    # the iteration counts of both solves are within 1 of the same-precision oracle
+    _, it_f, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5, 173,
+                        1, dtype, fused=recurrence == 1)
+    _, it_a, *_ = O.pcg(S.astype(dtype), P.astype(dtype), glam.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5,
+                        173, 1, dtype, fused=recurrence == 1)
+    assert abs(res.iters - it_f) <= 1 and abs(res.adj_iters - it_a) <= 1
+    # g_S is the masked, symmetrised outer product of the returned vectors
+    G = -0.5 * (np.outer(gg, lam) + np.outer(lam, gg))
+    expect = O.dense_to_cbtd(G, n, N)
+    mask = O.pattern_mask(n, N)
+    assert np.allclose(gS, expect * mask, rtol=1e-5 if dtype == np.float32 else 1e-12,
+                       atol=1e-6 * np.abs(expect).max())
+    assert not gS.reshape(N, 3, n * n)[0, 0].any() and not gS.reshape(N, 3, n * n)[-1, 2].any()
+    # error against the dense FP64 gradient (reported)
+    gg_d, gS_d = O.adjoint_dense(S, O.dense_solve(S, g, n, N) if N <= 64 else O.block_solve(S, g, n, N), glam, n, N) \
+        if N <= 64 else (None, None)
+    if gg_d is not None:
+        print(f"N={N} {dtype.__name__} rec={recurrence}: iters {res.iters}/{res.adj_iters} "
+              f"g_gamma err vs dense {rel(gg, gg_d):.2e}, g_S err {rel(gS, gS_d):.2e}")
+
+
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_adjoint_only_entry(gpu, recurrence):
+    """forward = 0: lambda is an input; same g_gamma/g_S as the fused launch."""
+    n, N, dtype = 14, 32, np.float64
+    S, P, g, glam = system(n, N, 77, dtype)
+    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=1e-10, max_iter=300, exit_mode=1,
+                                   recurrence=recurrence)
+    lam2, gg2, gS2, res2 = run_adjoint(S, P, g, glam, n, N, dtype, forward=False, lam_in=lam, tol=1e-10,
+                                       max_iter=300, exit_mode=1, recurrence=recurrence)
+    assert np.array_equal(lam2, lam)  # lambda untouched when it is an input
+    assert res2.adj_iters == res.adj_iters
+    assert np.allclose(gg2, gg, rtol=1e-12, atol=0) and np.allclose(gS2, gS, rtol=1e-12, atol=0)
+
+
+def test_adjoint_fd_and_dense_fp64(gpu):
+    """32 central finite differences (eps 1e-6) and the dense FP64 gradient."""
+    n, N, dtype = 14, 16, np.float64
+    S, P, g, glam = system(n, N, 4242, dtype)
+    kw = dict(tol=1e-13, max_iter=600, exit_mode=1, recurrence=1)
+    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, **kw)
+    gg_d, gS_d = O.adjoint_dense(S, O.dense_solve(S, g, n, N), glam, n, N)
+    assert rel(gg, gg_d) < 1e-8 and rel(gS, gS_d) < 1e-8
+    # finite differences of the exact FP64 solution map (a truncated iterative
+    # solve would put its own truncation error into the difference quotient)
+    def L(S_, g_):
+        return float(glam @ O.dense_solve(S_, g_, n, N))
+
+    rng = np.random.default_rng(5)
+    mask = O.pattern_mask(n, N)
+    eps = 1e-6
+    for k in range(32):
+        R = rng.normal(size=3 * n * n * N) * mask
+        D = O.cbtd_to_dense(R, n, N)
+        Dir = O.dense_to_cbtd(0.5 * (D + D.T), n, N)
+        fd = (L(S + eps * Dir, g) - L(S - eps * Dir, g)) / (2 * eps)
+        an = float(gS @ Dir)
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (k, fd, an)
+    dgam = rng.normal(size=n * N)
+    fd = (L(S, g + eps * dgam) - L(S, g - eps * dgam)) / (2 * eps)
+    assert abs(fd - float(gg @ dgam)) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_autograd_function(gpu):
+    import torch
+    from gbdpcg.autograd import pcg_solve
+    n, N = 6, 12
+    S, P, g, glam = system(n, N, 31, np.float64)
+    tS = _t(S, np.float64).requires_grad_(True)
+    tg = _t(g, np.float64).requires_grad_(True)
+    tP = _t(P, np.float64)
+    lam = pcg_solve(tS, tg, tP, n, N, tol=1e-13, max_iter=500)
+    loss = (lam * _t(glam, np.float64)).sum()
+    loss.backward()
+    gg_d, gS_d = O.adjoint_dense(S, O.dense_solve(S, g, n, N), glam, n, N)
+    assert rel(tg.grad.cpu().numpy(), gg_d) < 1e-9
+    assert rel(tS.grad.cpu().numpy(), gS_d) < 1e-9
+    assert torch.isfinite(tS.grad).all()

@@ -1,0 +1,140 @@
+"""GPU batched path: many independent systems of uneven (n, N) in one persistent
+launch, per-system tolerances, forward + adjoint, repeated runs and hipGraph
+replays.  Each system is checked against the oracle like a single solve.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a, dtype):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", torch.float64 if dtype == np.float64 else torch.float32)
+
+
+def make_batch(specs, dtype, adjoint=False, seed0=9000):
+    import gbdpcg
+    systems, host = [], []
+    for i, (n, N, tol) in enumerate(specs):
+        S, g = gbdpcg.lqr_generate(n, n // 2, N, seed0 + i, dtype=np.float64)
+        P = O.precond(S, n, N, 2)
+        glam = gbdpcg.randn(n * N, seed0 + 500 + i, dtype=np.float64)
+        d = dict(S=_t(S.astype(dtype), dtype), Pinv=_t(P.astype(dtype), dtype), gamma=_t(g.astype(dtype), dtype),
+                 lam=_t(np.zeros(n * N), dtype), n=n, N=N, tol=tol, max_iter=173)
+        if adjoint:
+            d.update(glam=_t(glam.astype(dtype), dtype), ggamma=_t(np.zeros(n * N), dtype),
+                     gS=_t(np.zeros(3 * n * n * N), dtype), adjoint=True)
+        systems.append(d)
+        host.append((S, P, g, glam))
+    return systems, host
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def check(systems, host, results, dtype, recurrence, adjoint=False):
+    tol = 1e-4 if dtype == np.float32 else 1e-10
+    for d, (S, P, g, glam), r in zip(systems, host, results):
+        n, N = d["n"], d["N"]
+        assert r.status == 0, (n, N, r.status)
+        _, it_same, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N,
+                               d["tol"], 173, 1, dtype, fused=recurrence == 1)
+        assert abs(r.iters - it_same) <= 1, (n, N, r.iters, it_same)
+        lam_ref, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 0.0, r.iters)
+        assert rel(d["lam"].cpu().numpy(), lam_ref) <= tol, (n, N)
+        if adjoint:
+            assert r.adj_status == 0
+            gg_ref, *_ = O.pcg(S, P, glam, np.zeros(n * N), n, N, 0.0, r.adj_iters)
+            gg = d["ggamma"].cpu().numpy().astype(np.float64)
+            assert rel(gg, gg_ref) <= tol
+            lam = d["lam"].cpu().numpy().astype(np.float64)
+            expect = O.dense_to_cbtd(-0.5 * (np.outer(gg, lam) + np.outer(lam, gg)), n, N)
+            assert np.allclose(d["gS"].cpu().numpy(), expect, rtol=1e-5, atol=1e-6 * np.abs(expect).max())
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_uneven_batch(gpu, dtype, recurrence):
+    import gbdpcg
+    rng = np.random.default_rng(3)
+    specs = [(int(rng.choice([6, 12, 14, 28])), int(rng.integers(16, 97)), float(rng.choice([1e-5, 1e-7])))
+             for _ in range(24)]
+    systems, host = make_batch(specs, dtype)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence)
+    info = b.info()
+    assert info["systems"] == 24 and info["grid"] > 0
+    b.run()
+    check(systems, host, b.results(), dtype, recurrence)
+
+
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_forward_adjoint_batch(gpu, recurrence):
+    import gbdpcg
+    specs = [(14, 32, 1e-5)] * 6 + [(6, 20, 1e-6)] * 3
+    systems, host = make_batch(specs, np.float32, adjoint=True)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence)
+    b.run()
+    check(systems, host, b.results(), np.float32, recurrence, adjoint=True)
+
+
+def test_repeated_runs_and_small_lds_budget(gpu):
+    """Runs continue the exchange epochs; a small LDS budget forces more workgroups
+    per system and several rounds per workgroup."""
+    import torch
+    import gbdpcg
+    specs = [(14, 64, 1e-5)] * 10 + [(28, 40, 1e-5)] * 4
+    systems, host = make_batch(specs, np.float32)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1, lds_budget=64 * 1024)
+    info = b.info()
+    assert info["lds_bytes"] <= 64 * 1024
+    first = None
+    for rep in range(3):
+        for d in systems:
+            d["lam"].zero_()
+        b.run()
+        res = b.results()
+        lams = [d["lam"].cpu().numpy().copy() for d in systems]
+        if first is None:
+            first = (lams, [r.iters for r in res])
+            check(systems, host, res, np.float32, 1)
+        else:
+            assert [r.iters for r in res] == first[1]
+            for a, c in zip(lams, first[0]):
+                assert np.array_equal(a, c)
+    torch.cuda.synchronize()
+
+
+def test_graph_capture_replay(gpu):
+    """The batch launch captured in a graph and replayed, with eager work and
+    synchronisations between replays; every replay matches the eager result."""
+    import torch
+    import gbdpcg
+    specs = [(14, 48, 1e-5)] * 8
+    systems, host = make_batch(specs, np.float32)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1)
+    b.run()
+    eager = [d["lam"].cpu().numpy().copy() for d in systems]
+    eager_iters = [r.iters for r in b.results()]
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    for d in systems:
+        d["lam"].zero_()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        for d in systems:
+            d["lam"].zero_()
+        b.run(stream=s)
+    for rep in range(4):
+        junk = torch.randn(1 << 16, device="cuda")
+        junk.mul_(2.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        res = b.results()
+        assert [r.iters for r in res] == eager_iters
+        for d, e in zip(systems, eager):
+            assert np.array_equal(d["lam"].cpu().numpy(), e)
