@@ -47,6 +47,9 @@ struct gbdpcg_batch {
   int *d_err = nullptr;
   gbdpcg_batch_info info{};
   size_t slot_words = 0;
+  bool reg = false;  // register-resident kernel
+  int kn = 0;        // its stateSize specialisation (0 = runtime dispatch)
+  int nt = 256;      // threads per workgroup
 };
 
 namespace {
@@ -115,11 +118,41 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   const size_t max_lds = prop.sharedMemPerBlock;
   const size_t budget = lds_budget_bytes ? std::min<size_t>(lds_budget_bytes, max_lds) : max_lds;
 
+  // kernel: register-resident rows when every system suits it (fused recurrence,
+  // stateSize in the specialised set, at least 4 knots), else LDS-resident rows
+  bool use_reg = fused_cfg && cfg->kernel != GBDPCG_KERNEL_LDS;
+  bool same_n = true;
+  for (uint32_t i = 0; i < count; ++i) {
+    if (!reg_supported((int)systems[i].n) || systems[i].N < 4) use_reg = false;
+    // FP64 rows of n = 28 do not fit the registers of a 512-thread workgroup
+    if (dtype == GBDPCG_F64 && systems[i].n > 14) use_reg = false;
+    if (systems[i].n != systems[0].n) same_n = false;
+  }
+  if (cfg->kernel == GBDPCG_KERNEL_REGISTER && !use_reg) {
+    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28} and N >= 4 for every system");
+    return GBDPCG_ERR_UNSUPPORTED;
+  }
+  const int nt = (use_reg && same_n && cfg->block[0] == 256) ? 256 : 512;
+  const int kn = same_n ? (int)systems[0].n : 0;
+
   // workgroups per system and the launch's LDS size
   std::vector<uint32_t> G(count);
   std::vector<uint8_t> fused(count);
   size_t lds = 0;
-  for (uint32_t i = 0; i < count; ++i) {
+  for (uint32_t i = 0; use_reg && i < count; ++i) {
+    const gbdpcg_system_desc &d = systems[i];
+    const uint32_t kmax = reg_max_knots(d.n, nt);
+    uint32_t g = d.wgs ? d.wgs : (d.N + kmax - 1) / kmax;
+    if (g == 0 || (d.N + g - 1) / g > kmax || d.N / g < 2 || g > (uint32_t)kMaxWgsPerSystem) {
+      set_error("system %u: %u workgroups do not suit the register kernel (N=%u, 2..%u knots each)", i, g, d.N,
+                kmax);
+      return GBDPCG_ERR_TOO_LARGE;
+    }
+    G[i] = g;
+    fused[i] = 1;
+    lds = std::max(lds, reg_lds_bytes(dtype, d.n, (d.N + g - 1) / g, nt));
+  }
+  for (uint32_t i = 0; !use_reg && i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
     const uint32_t kmax = max_knots(d.n, d.N, ts, budget);
     if (kmax == 0) {
@@ -140,9 +173,14 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     lds = std::max(lds, item_lds_bytes(d.n, (d.N + g - 1) / g, g, ts));
   }
   lds = (lds + 255) & ~(size_t)255;
-  HIPCHK(pcg_prepare(dtype, lds));
   int occ = 0;
-  HIPCHK(pcg_occupancy(dtype, lds, &occ));
+  if (use_reg) {
+    HIPCHK(reg_prepare(dtype, kn, nt, lds));
+    HIPCHK(reg_occupancy(dtype, kn, nt, lds, &occ));
+  } else {
+    HIPCHK(pcg_prepare(dtype, lds));
+    HIPCHK(pcg_occupancy(dtype, lds, &occ));
+  }
   if (occ <= 0) {
     set_error("no workgroup fits at %zu B LDS", lds);
     return GBDPCG_ERR_TOO_LARGE;
@@ -279,6 +317,9 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   plan->info.rounds = rounds;
   plan->info.wgs_per_cu = (uint32_t)occ;
   plan->info.lds_bytes = (uint32_t)lds;
+  plan->reg = use_reg;
+  plan->kn = kn;
+  plan->nt = use_reg ? nt : kThreads;
   plan->info.systems = count;
   plan->info.total_items = total_items;
   plan->info.est_cost = makespan;
@@ -293,6 +334,11 @@ int gbdpcg_batch_run(gbdpcg_batch *plan, void *stream) {
   }
   KSys dummy;
   memset(&dummy, 0, sizeof dummy);
+  if (plan->reg) {
+    HIPCHK(reg_launch(plan->dtype, plan->kn, plan->nt, dummy, plan->d_sys, plan->d_items, plan->info.rounds, 1,
+                      plan->info.grid, plan->info.lds_bytes, static_cast<hipStream_t>(stream)));
+    return GBDPCG_OK;
+  }
   HIPCHK(pcg_launch(plan->dtype, dummy, plan->d_sys, plan->d_items, plan->info.rounds, 1, plan->info.grid,
                     plan->info.lds_bytes, static_cast<hipStream_t>(stream)));
   return GBDPCG_OK;

@@ -538,6 +538,24 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
 
 }  // namespace
 
+// NS != 0: the kernel is specialised for stateSize NS.  NS == 0: runtime
+// dispatch over the specialised sizes (batches of uneven stateSize).
+template <typename T, int NS, int NT>
+__device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *lds) {
+  if constexpr (NS != 0) {
+    reg_item<T, NS, NT>(d, rank, lds);
+  } else {
+    switch (d.n) {
+      case 2: reg_item<T, 2, NT>(d, rank, lds); break;
+      case 6: reg_item<T, 6, NT>(d, rank, lds); break;
+      case 12: reg_item<T, 12, NT>(d, rank, lds); break;
+      case 14: reg_item<T, 14, NT>(d, rank, lds); break;
+      case 28: reg_item<T, 28, NT>(d, rank, lds); break;
+      default: break;  // the planner only admits the sizes above
+    }
+  }
+}
+
 // Single system (systems == nullptr): workgroup b is rank b/stride when
 // b % stride == 0.  Batched: items[round * gridDim.x + b] as in pcg_kernel.
 template <typename T, int NS, int NT>
@@ -550,7 +568,7 @@ __global__ void __launch_bounds__(NT) pcg_reg_kernel(KSys one, const KSys *__res
     if (blockIdx.x % stride) return;
     const uint32_t rank = blockIdx.x / stride;
     if (rank >= one.G) return;
-    reg_item<T, NS, NT>(one, rank, lds);
+    reg_dispatch<T, NS, NT>(one, rank, lds);
     return;
   }
   for (uint32_t rd = 0; rd < rounds; ++rd) {
@@ -558,7 +576,7 @@ __global__ void __launch_bounds__(NT) pcg_reg_kernel(KSys one, const KSys *__res
     if (item == kNoItem) continue;
     __syncthreads();
     const KSys d = systems[item >> 12];
-    reg_item<T, NS, NT>(d, item & 0xFFFu, lds);
+    reg_dispatch<T, NS, NT>(d, item & 0xFFFu, lds);
   }
 }
 
@@ -580,6 +598,9 @@ const void *reg_ptr_n(int n) {
     case 12: return reg_ptr<T, 12, NT>();
     case 14: return reg_ptr<T, 14, NT>();
     case 28: return reg_ptr<T, 28, NT>();
+    case 0:  // mixed sizes (batches): 512 threads only
+      if constexpr (NT == 512) return reg_ptr<T, 0, NT>();
+      return nullptr;
     default: return nullptr;
   }
 }
