@@ -39,6 +39,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the solver runs each system on one "
@@ -121,8 +123,17 @@ def main():
     ts = 4
     hbm_bytes = (2 * 3 * n * n * N + 2 * n * N) * ts
     achieved = hbm_bytes / (kern_us * 1e-6) / 1e9
+    # measured HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
+    # same command, scaled by the counters' reading of a plain copy of known size
+    traffic = None
+    counters = os.path.join(ROOT, "profiles", "pcg_reg_kernel_n14_N128_counters.json")
+    if os.path.exists(counters):
+        with open(counters) as f:
+            rec = json.load(f)["kernels"].get("gbdpcg::pcg_reg_kernel<float, 14, 256>")
+        if rec:
+            traffic = int(rec["traffic_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "algorithmic_bytes": hbm_bytes,
                 "note": "latency-bound: one cross-workgroup exchange per iteration; see per_iteration_us"}
 
     # batched forward + adjoint (64 systems, same shape), one persistent launch
@@ -166,6 +177,16 @@ def main():
                    "mean_iters_adj": float(np.mean([r.adj_iters for r in bres])),
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
                    "plan": plan.info()}
+
+    if args.calibrate_copy:
+        # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch give the
+        # counters' scale (256 MiB read, 256 MiB written)
+        src = torch.ones(64 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     # CPU baseline: FP64 host restatement of the reference iteration (oracle/), one thread
     from oracle import oracle as O
