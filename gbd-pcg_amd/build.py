@@ -9,6 +9,7 @@ import sys
 from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "gbdpcg")
 LIB = os.path.join(OUT_DIR, "libgbdpcg.so")
@@ -54,6 +55,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs], verbose)
     os.replace(tmp, LIB)
+    # the C++ drop-in example: host-only C++ against include/ and the library
+    ex = os.path.join(OUT_DIR, "pcg_solve_example")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(HERE, "examples", "pcg_solve.cpp"), "-o", ex + ".tmp", "-L" + OUT_DIR, "-lgbdpcg",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(ex + ".tmp", ex)
     return LIB
 
 
