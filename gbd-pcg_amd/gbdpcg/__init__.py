@@ -267,6 +267,29 @@ def solve_host(S: np.ndarray, gamma: np.ndarray, lam: np.ndarray, n: int, N: int
     return check(f(_np_ptr(S), _np_ptr(Pinv), _np_ptr(gamma), _np_ptr(lam), n, N, ctypes.byref(cfg)))
 
 
+def solve_csr(S_csr, gamma: np.ndarray, lam: np.ndarray, n: int, N: int, P_csr=None, *, tol=1e-6, max_iter=25,
+              exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, precond=PRECOND_SS, wgs=0) -> int:
+    """CSR overload (reference include/interface.cuh:8-20): S_csr / P_csr are
+    (row_ptr, col_ind, val) triples of an nN x nN matrix; lam (numpy) in/out."""
+    def parts(m):
+        if m is None:
+            return None, None, None
+        rp = np.ascontiguousarray(m[0], dtype=np.uint32)
+        ci = np.ascontiguousarray(m[1], dtype=np.uint32)
+        v = np.ascontiguousarray(m[2], dtype=lam.dtype)
+        return rp, ci, v
+    sr, sc, sv = parts(S_csr)
+    pr, pc, pv = parts(P_csr)
+    gamma = np.ascontiguousarray(gamma, dtype=lam.dtype)
+    if not lam.flags["C_CONTIGUOUS"]:
+        raise ValueError("lam must be C-contiguous")
+    dt = _dtype_code(lam.dtype)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, None)
+    f = lib().gbdpcg_solve_csr_f64 if dt == F64 else lib().gbdpcg_solve_csr_f32
+    return check(f(_np_ptr(sr), _np_ptr(sc), _np_ptr(sv), _np_ptr(pr), _np_ptr(pc), _np_ptr(pv), _np_ptr(gamma),
+                   _np_ptr(lam), n, N, ctypes.byref(cfg)))
+
+
 def precond_host(S: np.ndarray, n: int, N: int, kind: int = PRECOND_SS) -> np.ndarray:
     P = np.zeros_like(S)
     check(lib().gbdpcg_precond_host(_dtype_code(S.dtype), n, N, _np_ptr(S), _np_ptr(P), kind))

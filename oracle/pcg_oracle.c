@@ -67,15 +67,18 @@ static int lu_factor(int m, double *a, int *piv) {
   return 0;
 }
 
-/* solves LU x = b for `cols` right-hand sides stored row-major in b (m x cols) */
+/* solves LU x = b for `cols` right-hand sides stored row-major in b (m x cols).
+ * lu_factor swaps whole rows (the stored multipliers move with them), so all the
+ * interchanges are applied to b first, then plain forward substitution. */
 static void lu_solve(int m, const double *lu, const int *piv, double *b, int cols) {
-  for (int c = 0; c < m; ++c) {
+  for (int c = 0; c < m; ++c)
     if (piv[c] != c)
       for (int j = 0; j < cols; ++j) {
         const double t = b[(long)c * cols + j];
         b[(long)c * cols + j] = b[(long)piv[c] * cols + j];
         b[(long)piv[c] * cols + j] = t;
       }
+  for (int c = 0; c < m; ++c) {
     for (int r = c + 1; r < m; ++r)
       for (int j = 0; j < cols; ++j) b[(long)r * cols + j] -= lu[(long)r * m + c] * b[(long)c * cols + j];
   }

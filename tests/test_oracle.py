@@ -197,6 +197,34 @@ def test_block_solve_matches_dense(n, N):
     assert np.allclose(O.block_solve(S, b, n, N), np.linalg.solve(D, b), rtol=1e-10, atol=1e-12)
 
 
+@pytest.mark.parametrize("N", [6, 16])
+def test_block_solve_with_pivoting(N):
+    """Blocks whose LU needs row interchanges at several steps (tiny leading
+    diagonal, nonsymmetric): a solver that applies the interchanges out of order
+    fails this, while diagonally dominant SPD blocks never pivot."""
+    n = 5
+    rng = np.random.default_rng(N)
+    D = np.zeros((n * N, n * N))
+    for k in range(N):
+        B = rng.normal(size=(n, n))
+        B[np.arange(n), np.arange(n)] *= 1e-3
+        D[k * n:(k + 1) * n, k * n:(k + 1) * n] = B + 0.5 * np.roll(np.eye(n), 1, axis=1) * n
+        if k > 0:
+            D[k * n:(k + 1) * n, (k - 1) * n:k * n] = 0.2 * rng.normal(size=(n, n))
+            D[(k - 1) * n:k * n, k * n:(k + 1) * n] = 0.2 * rng.normal(size=(n, n))
+    S = O.dense_to_cbtd(D, n, N)
+    b = rng.normal(size=n * N)
+    assert np.allclose(O.block_solve(S, b, n, N), np.linalg.solve(D, b), rtol=1e-9, atol=1e-11)
+
+
+def test_block_solve_lqr_system():
+    import gbdpcg
+    S, g = gbdpcg.lqr_generate(14, 7, 16, 2128, dtype=np.float64)
+    D = O.cbtd_to_dense(S, 14, 16)
+    x = O.block_solve(S, g, 14, 16)
+    assert np.linalg.norm(D @ x - g) <= 1e-12 * np.linalg.norm(g)
+
+
 def test_block_solve_negative_definite():
     d, S, _, g = readme()
     assert np.allclose(O.block_solve(S, g, 2, 3), d["lambda_closed_form_survey"], rtol=1e-8)
