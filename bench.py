@@ -119,9 +119,10 @@ def main():
     ref_us = t_start.elapsed_time(t_end) * 1000.0 / nref
     lams[:W + nref].zero_()
 
-    # roofline of the solve kernel: algorithmic HBM bytes per launch
+    # roofline of the solve kernel: algorithmic HBM bytes per launch (S and Pinv read
+    # once, gamma and lambda0 read, lambda written)
     ts = 4
-    hbm_bytes = (2 * 3 * n * n * N + 2 * n * N) * ts
+    hbm_bytes = (2 * 3 * n * n * N + 3 * n * N) * ts
     achieved = hbm_bytes / (kern_us * 1e-6) / 1e9
     # measured HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
     # same command, scaled by the counters' reading of a plain copy of known size

@@ -3,9 +3,10 @@
 // Same method and exchange protocol as pcg_kernel.hip, in the single-reduction
 // (Chronopoulos-Gear) form, with every block row a workgroup owns held in
 // registers instead of LDS:
-//   * a quad of lanes owns one matrix row; each lane keeps a CW-column chunk of
-//     that row of Pinv (rows of the owned knots plus one ghost knot each side)
-//     and of S (owned knots) in VGPRs for the whole solve, loaded from HBM once;
+//   * LPR lanes (reg_lpr: 1 for n <= 2, else 2) share one matrix row; each lane
+//     keeps a CW-column chunk of that row of Pinv (rows of the owned knots plus
+//     one ghost knot each side) and of S (owned knots) in VGPRs for the whole
+//     solve, loaded from HBM once;
 //   * per iteration: one pass u = Pinv r (owned + ghost rows), one pass w = S u
 //     (owned rows), then one cross-workgroup exchange carrying (r'u, w'u, r'r)
 //     and the two edge knots of w each side;

@@ -10,6 +10,8 @@
 
 #include "gpu_pcg.hpp"
 
+using namespace gbdpcg;  // the reference declares these names globally
+
 template <typename T>
 static int readme_example(const char *name) {
   const uint32_t state_size = 2, knot_points = 3;
@@ -20,8 +22,8 @@ static int readme_example(const char *name) {
   for (int i = 0; i < 36; i++) h_Pinv[i] = h_S[i] == 0 ? T(0) : T(1) / h_S[i];
   T h_gamma[6] = {T(3.1385), 0, 0, T(3.0788), T(.0031), T(3.0788)};
   T h_lambda[6] = {0, 0, 0, 0, 0, 0};
-  gbdpcg::pcg_config<T> config;
-  const int iters = gbdpcg::solvePCG(h_S, h_Pinv, h_gamma, h_lambda, state_size, knot_points, &config);
+  struct pcg_config<T> config;  // the reference example, examples/pcg_solve.cu:34-41
+  const int iters = solvePCG<T>(h_S, h_Pinv, h_gamma, h_lambda, state_size, knot_points, &config);
   if (iters < 0) {
     printf("%s error %d: %s\n", name, iters, gbdpcg_last_error());
     return 1;
@@ -38,10 +40,10 @@ static int lqr_example() {
   if (gbdpcg_lqr_generate(GBDPCG_F64, n, m, N, 2128, 0.1, S.data(), gamma.data(), nullptr, nullptr, nullptr,
                           nullptr) != GBDPCG_OK)
     return 1;
-  gbdpcg::cbtd_t<double> cb{S.data(), nullptr};
-  gbdpcg::pcg_config<double> config(1e-10, 500);
+  cbtd_t<double> cb{S.data(), nullptr};
+  pcg_config<double> config(1e-10, 500);
   config.exit_mode = GBDPCG_EXIT_REL_RESID;
-  const int iters = gbdpcg::pcg_solve(&cb, gamma.data(), lambda.data(), n, N, &config);
+  const int iters = pcg_solve(&cb, gamma.data(), lambda.data(), n, N, &config);
   if (iters < 0) {
     printf("lqr error %d: %s\n", iters, gbdpcg_last_error());
     return 1;

@@ -115,16 +115,17 @@ struct dt<double> {
 };
 }  // namespace detail
 
-// Host pointers.  Returns the iteration count, or a negative GBDPCG_ERR_*.
-inline int solvePCG(float *h_S, float *h_Pinv, float *h_gamma, float *h_lambda, unsigned stateSize,
-                    unsigned knotPoints, pcg_config<float> *config) {
+// Host pointers, called as the reference example does (solvePCG<float>(h_S, ...),
+// examples/pcg_solve.cu:35-41).  Returns the iteration count, or a negative
+// GBDPCG_ERR_*.
+template <typename T>
+inline int solvePCG(T *h_S, T *h_Pinv, T *h_gamma, T *h_lambda, unsigned stateSize, unsigned knotPoints,
+                    pcg_config<T> *config) {
   const gbdpcg_config c = detail::to_c(config);
-  return gbdpcg_solve_f32(h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, &c);
-}
-inline int solvePCG(double *h_S, double *h_Pinv, double *h_gamma, double *h_lambda, unsigned stateSize,
-                    unsigned knotPoints, pcg_config<double> *config) {
-  const gbdpcg_config c = detail::to_c(config);
-  return gbdpcg_solve_f64(h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, &c);
+  if constexpr (detail::dt<T>::code == GBDPCG_F64)
+    return gbdpcg_solve_f64(h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, &c);
+  else
+    return gbdpcg_solve_f32(h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, &c);
 }
 
 // Device pointers (the reference's scratch vectors are accepted and not used).
