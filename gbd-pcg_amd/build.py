@@ -23,7 +23,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
 HOST_FLAGS = ["--offload-arch=" + ARCH, "-O2", "-std=c++17", "-fPIC", "-Wall"]
 
 HIP_SOURCES = ["pcg_kernel.hip", "pcg_reg_kernel.hip", "aux_kernels.hip"]
-CPP_SOURCES = ["api.cpp", "batch.cpp", "problem.cpp"]
+CPP_SOURCES = ["api.cpp", "batch.cpp", "problem.cpp", "tune.cpp"]
 
 
 def _run(cmd, verbose):

@@ -209,7 +209,7 @@ static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 // dominated by the two cross-workgroup exchanges, whose latency grows slowly with
 // the number of workgroups, while the on-chip matvec grows with the knots per
 // workgroup), all workgroups on one XCD when they fit there.
-static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t want_G, Layout *L) {
+static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t want_G, uint32_t kcap, Layout *L) {
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
@@ -218,6 +218,7 @@ static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t w
     set_error("fused recurrence needs at least 2 knots per workgroup");
     return GBDPCG_ERR_INVALID;
   }
+  if (!want_G && kcap) want_G = std::min(ceil_div(N, kcap), maxG);
   uint32_t G = want_G ? want_G : std::min<uint32_t>(std::max<uint32_t>(ceil_div(N, 8), 1), 32);
   if (G > maxG) G = maxG;
   while (item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype)) > di.max_lds && G < maxG && !want_G) ++G;
@@ -252,20 +253,22 @@ static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t w
 // per SIMD: every wave runs the uniform scalar work, so fewer waves cost less)
 // with up to 8 knots per workgroup; 512 threads when the knots need it.
 // Returns GBDPCG_ERR_UNSUPPORTED when the shape does not suit this kernel.
-static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, Layout *L) {
+static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, uint32_t kcap, Layout *L,
+                           int only_nt = 0) {
   if (!reg_supported((int)n) || N < 4) return GBDPCG_ERR_UNSUPPORTED;
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
   for (int nt : {256, 512}) {
+    if (only_nt && nt != only_nt) continue;
     // FP64 rows of n = 28 need ~176 VGPRs of matrix chunks: one wave per SIMD only
     if (dtype == GBDPCG_F64 && n > 14 && nt > 256) continue;
     const uint32_t kmax = reg_max_knots(n, nt);
     if (kmax < 2) continue;
     uint32_t G = want_G;
-    if (!G) {
-      const uint32_t target = std::min<uint32_t>(kmax, 8);
-      G = ceil_div(N, target);
+    if (!G) {  // about 8 knots (or the cap) per workgroup, never fewer than 2
+      const uint32_t target = std::min<uint32_t>(kmax, kcap ? std::max<uint32_t>(kcap, 2u) : 8u);
+      G = std::min(ceil_div(N, target), N / 2);
     }
     if (G < 1 || N / G < 2 || ceil_div(N, G) > kmax || G > (uint32_t)kMaxWgsPerSystem) continue;
     const size_t lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
@@ -284,6 +287,67 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, L
     return GBDPCG_OK;
   }
   return GBDPCG_ERR_UNSUPPORTED;
+}
+
+// Layout of a single solve: the autotuner's choice for this shape when the config
+// leaves everything automatic, else the planners (register kernel first when the
+// recurrence is fused).
+static std::string single_key(int dev, int dtype, uint32_t n, uint32_t N, bool fused, int exit_mode) {
+  return "single:" + device_tag(dev) + ":" + std::to_string(dtype) + ":" + std::to_string(n) + ":" +
+         std::to_string(N) + ":" + std::to_string(fused ? 1 : 0) + ":" + std::to_string(exit_mode);
+}
+
+// Rebuilds a layout from its (kernel, threads, workgroups, stride); checks that it
+// still fits the device.
+static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint32_t G, uint32_t stride,
+                       Layout *L) {
+  DeviceInfo di;
+  int rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  if (G == 0 || G > N || (stride != 1 && stride != (uint32_t)kXcds)) return GBDPCG_ERR_UNSUPPORTED;
+  Layout t;
+  t.G = G;
+  t.stride = stride;
+  t.grid = G * stride;
+  t.reg = reg;
+  t.nt = reg ? nt : kThreads;
+  int occ = 0;
+  if (reg) {
+    if (!reg_supported((int)n) || ceil_div(N, G) > reg_max_knots(n, nt) || N / G < 2)
+      return GBDPCG_ERR_UNSUPPORTED;
+    t.lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
+    if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
+    HIPCHK(reg_prepare(dtype, (int)n, nt, t.lds));
+    HIPCHK(reg_occupancy(dtype, (int)n, nt, t.lds, &occ));
+  } else {
+    t.lds = item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype));
+    if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
+    rc = occupancy(dtype, t.lds, &occ);
+    if (rc < 0) return rc;
+  }
+  const uint32_t per_xcd = (uint32_t)(di.cus / kXcds) * (uint32_t)occ;
+  if (occ <= 0 || G > (uint32_t)di.cus * (uint32_t)occ || (stride == (uint32_t)kXcds && G > per_xcd))
+    return GBDPCG_ERR_UNSUPPORTED;
+  *L = t;
+  return GBDPCG_OK;
+}
+
+static int choose_layout(int dev, int dtype, uint32_t n, uint32_t N, bool fused, const gbdpcg_config *cfg,
+                         Layout *L) {
+  if (cfg->grid[0] == 0 && cfg->knots_per_wg == 0 && cfg->kernel == GBDPCG_KERNEL_AUTO) {
+    TuneChoice c;
+    if (tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
+        layout_from(dtype, n, N, c.a != 0, (int)c.b, c.c, c.d, L) == GBDPCG_OK)
+      return GBDPCG_OK;
+  }
+  int rc = GBDPCG_ERR_UNSUPPORTED;
+  if (fused && cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L);
+  if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_REGISTER) {
+    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28}, N >= 4 and a fitting layout");
+    return GBDPCG_ERR_UNSUPPORTED;
+  }
+  if (rc == GBDPCG_ERR_UNSUPPORTED) rc = plan_single(dtype, n, N, fused, cfg->grid[0], cfg->knots_per_wg, L);
+  return rc;
 }
 
 static int validate_cfg(const gbdpcg_config *cfg) {
@@ -360,13 +424,7 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
   bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED;
   if (fused && N < 2) fused = false;  // one knot: the reference recurrence
   Layout L;
-  rc = GBDPCG_ERR_UNSUPPORTED;
-  if (fused && cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, cfg->grid[0], &L);
-  if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_REGISTER) {
-    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28}, N >= 4 and a fitting layout");
-    return GBDPCG_ERR_UNSUPPORTED;
-  }
-  if (rc == GBDPCG_ERR_UNSUPPORTED) rc = plan_single(dtype, n, N, fused, cfg->grid[0], &L);
+  rc = choose_layout(dev, dtype, n, N, fused, cfg, &L);
   if (rc < 0) return rc;
 
   if (Pinv == nullptr) {
@@ -503,7 +561,7 @@ int gbdpcg_check_occupancy(int dtype, uint32_t n, uint32_t N, uint32_t *max_wgs)
     return GBDPCG_ERR_INVALID;
   }
   Layout L;
-  int rc = plan_single(dtype, n, N, false, 0, &L);
+  int rc = plan_single(dtype, n, N, false, 0, 0, &L);
   if (rc < 0) return rc;
   int occ = 0;
   rc = occupancy(dtype, L.lds, &occ);
@@ -569,6 +627,173 @@ int gbdpcg_solve_csr_f64(const uint32_t *Sr, const uint32_t *Sc, const double *S
                          const uint32_t *Pc, const double *Pv, const double *h_gamma, double *h_lambda, uint32_t n,
                          uint32_t N, const gbdpcg_config *cfg) {
   return solve_csr_impl<double>(GBDPCG_F64, Sr, Sc, Sv, Pr, Pc, Pv, h_gamma, h_lambda, n, N, cfg);
+}
+
+int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const void *d_Pinv, const void *d_gamma,
+                       const gbdpcg_config *cfg, uint32_t trials, gbdpcg_tune_report *report) {
+  int rc = validate_cfg(cfg);
+  if (rc < 0) return rc;
+  if (!valid_dtype(dtype) || n == 0 || N == 0 || !d_S || !d_gamma) {
+    set_error("gbdpcg_tune_single: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  if (trials == 0) trials = 5;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipStream_t st = static_cast<hipStream_t>(cfg->stream);
+  if (capturing(st)) {
+    set_error("gbdpcg_tune_single: cannot tune inside graph capture");
+    return GBDPCG_ERR_INVALID;
+  }
+  const bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED && N >= 2;
+  const size_t ts = tsize(dtype);
+
+  // candidates: the untuned default first, then every (kernel, threads, knots per
+  // workgroup, XCD placement) that fits
+  std::vector<Layout> cands;
+  auto add = [&](const Layout &L) {
+    for (const Layout &c : cands)
+      if (c.reg == L.reg && c.nt == L.nt && c.G == L.G && c.stride == L.stride) return;
+    cands.push_back(L);
+  };
+  {
+    Layout def;
+    rc = GBDPCG_ERR_UNSUPPORTED;
+    if (fused && cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, 0, 0, &def);
+    if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel != GBDPCG_KERNEL_REGISTER)
+      rc = plan_single(dtype, n, N, fused, 0, 0, &def);
+    if (rc < 0) return rc;
+    add(def);
+  }
+  const uint32_t Ks[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64};
+  for (int reg = 1; reg >= 0; --reg) {
+    if (reg && !(fused && cfg->kernel != GBDPCG_KERNEL_LDS)) continue;
+    if (!reg && cfg->kernel == GBDPCG_KERNEL_REGISTER) continue;
+    for (int nt : {256, 512}) {
+      if (!reg && nt != kThreads) continue;
+      for (uint32_t K : Ks) {
+        const uint32_t G = ceil_div(N, K);
+        if (G > (uint32_t)kMaxWgsPerSystem || (fused && N / G < 2)) continue;
+        for (uint32_t stride : {(uint32_t)kXcds, 1u}) {
+          Layout L;
+          if (layout_from(dtype, n, N, reg != 0, nt, G, stride, &L) == GBDPCG_OK) add(L);
+        }
+      }
+    }
+  }
+
+  // scratch: lambda (zeroed before every run), Pinv if not given, records, result, error words
+  size_t max_words = 0;
+  for (const Layout &L : cands) max_words = std::max(max_words, 2 * (size_t)L.G * rec_words(n, (uint32_t)(ts / 4)));
+  void *lam = nullptr, *pinv = nullptr;
+  uint64_t *slots = nullptr;
+  gbdpcg_result *res = nullptr;
+  int *err = nullptr;
+  hipEvent_t ea = nullptr, eb = nullptr;
+  auto cleanup = [&]() {
+    if (lam) (void)hipFree(lam);
+    if (pinv) (void)hipFree(pinv);
+    if (slots) (void)hipFree(slots);
+    if (res) (void)hipFree(res);
+    if (err) (void)hipFree(err);
+    if (ea) (void)hipEventDestroy(ea);
+    if (eb) (void)hipEventDestroy(eb);
+  };
+  auto fail = [&](const char *what, hipError_t e) {
+    set_error("gbdpcg_tune_single: %s failed: %s", what, hipGetErrorString(e));
+    cleanup();
+    return GBDPCG_ERR_DEVICE;
+  };
+  hipError_t e;
+  if ((e = hipMalloc(&lam, (size_t)n * N * ts)) != hipSuccess) return fail("hipMalloc", e);
+  if ((e = hipMalloc(&slots, max_words * sizeof(uint64_t))) != hipSuccess) return fail("hipMalloc", e);
+  if ((e = hipMalloc(&res, sizeof(gbdpcg_result))) != hipSuccess) return fail("hipMalloc", e);
+  if ((e = hipMalloc(&err, 2 * sizeof(int))) != hipSuccess) return fail("hipMalloc", e);
+  if ((e = hipEventCreate(&ea)) != hipSuccess || (e = hipEventCreate(&eb)) != hipSuccess)
+    return fail("hipEventCreate", e);
+  if ((e = hipMemsetAsync(err, 0, 2 * sizeof(int), st)) != hipSuccess) return fail("hipMemsetAsync", e);
+  const void *P = d_Pinv;
+  if (!P) {
+    if ((e = hipMalloc(&pinv, (size_t)3 * n * n * N * ts)) != hipSuccess) return fail("hipMalloc", e);
+    if ((e = precond_launch(dtype, n, N, d_S, pinv, cfg->precond, err + 1, st)) != hipSuccess)
+      return fail("preconditioner launch", e);
+    P = pinv;
+  }
+
+  double best_us = 1e300, default_us = -1.0;
+  size_t best = 0;
+  uint32_t timed = 0;
+  for (size_t ci = 0; ci < cands.size(); ++ci) {
+    const Layout &L = cands[ci];
+    if ((e = hipMemsetAsync(slots, 0, max_words * sizeof(uint64_t), st)) != hipSuccess)
+      return fail("hipMemsetAsync", e);
+    KSys one{};
+    one.S = d_S;
+    one.Pinv = P;
+    one.gamma = d_gamma;
+    one.lambda = lam;
+    one.slots = slots;
+    one.result = res;
+    one.err = err;
+    one.tol = cfg->exit_tol;
+    one.n = n;
+    one.N = N;
+    one.G = L.G;
+    one.max_iter = cfg->max_iter;
+    one.flags = KF_FWD | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u);
+    double total = 0.0;
+    for (uint32_t t = 0; t <= trials; ++t) {
+      if ((e = hipMemsetAsync(lam, 0, (size_t)n * N * ts, st)) != hipSuccess) return fail("hipMemsetAsync", e);
+      if ((e = hipEventRecord(ea, st)) != hipSuccess) return fail("hipEventRecord", e);
+      e = L.reg ? reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st)
+                : pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st);
+      if (e != hipSuccess) return fail("solve launch", e);
+      if ((e = hipEventRecord(eb, st)) != hipSuccess) return fail("hipEventRecord", e);
+      if ((e = hipEventSynchronize(eb)) != hipSuccess) return fail("hipEventSynchronize", e);
+      float ms = 0.f;
+      if ((e = hipEventElapsedTime(&ms, ea, eb)) != hipSuccess) return fail("hipEventElapsedTime", e);
+      if (t > 0) total += ms * 1000.0;
+    }
+    int herr[2] = {0, 0};
+    if ((e = hipMemcpy(herr, err, sizeof herr, hipMemcpyDeviceToHost)) != hipSuccess) return fail("hipMemcpy", e);
+    if (herr[1]) {
+      cleanup();
+      set_error("gbdpcg_tune_single: preconditioner: a diagonal block of S is singular");
+      return GBDPCG_ERR_INVALID;
+    }
+    if (herr[0]) {  // a wait timed out: not a usable layout
+      if ((e = hipMemset(err, 0, sizeof(int))) != hipSuccess) return fail("hipMemset", e);
+      continue;
+    }
+    ++timed;
+    const double us = total / trials;
+    if (ci == 0) default_us = us;
+    if (us < best_us) {
+      best_us = us;
+      best = ci;
+    }
+  }
+  cleanup();
+  if (timed == 0) {
+    set_error("gbdpcg_tune_single: no candidate layout completed");
+    return GBDPCG_ERR_TIMEOUT;
+  }
+  const Layout &B = cands[best];
+  tune_store(single_key(dev, dtype, n, N, fused, cfg->exit_mode),
+             TuneChoice{B.reg ? 1u : 0u, (uint32_t)B.nt, B.G, B.stride});
+  if (report) {
+    gbdpcg_tune_report r{};
+    r.candidates = timed;
+    r.from_cache = 0;
+    r.knots_per_wg = ceil_div(N, B.G);
+    r.threads = (uint32_t)B.nt;
+    r.wgs = B.G;
+    r.kernel = B.reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+    r.best_us = best_us;
+    r.default_us = default_us;
+    *report = r;
+  }
+  return GBDPCG_OK;
 }
 
 int gbdpcg_precond_device(int dtype, uint32_t n, uint32_t N, const void *d_S, void *d_Pinv, int kind,

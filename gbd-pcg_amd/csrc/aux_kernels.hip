@@ -262,8 +262,7 @@ hipError_t schur_launch(int dtype, uint32_t n, uint32_t m, uint32_t N, const voi
 // has written (the shift moves values towards lower indices), so the in-place
 // shift is race free.
 template <typename T>
-__global__ void __launch_bounds__(kAuxThreads) warm_start_kernel(T *lambda, T *gamma, const T *noise, int n, int N) {
-  const int total = n * N;
+__device__ __forceinline__ void shift_knots(T *lambda, int n, int N) {
   const int limit = n * (N - 1);
   for (int base = 0; base < limit; base += kAuxThreads) {
     const int i = base + threadIdx.x;
@@ -273,8 +272,30 @@ __global__ void __launch_bounds__(kAuxThreads) warm_start_kernel(T *lambda, T *g
     if (i < limit) lambda[i] = v;
     __syncthreads();
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kAuxThreads) warm_start_kernel(T *lambda, T *gamma, const T *noise, int n, int N) {
+  shift_knots(lambda, n, N);
   if (noise != nullptr) {
-    for (int i = threadIdx.x; i < total; i += kAuxThreads) gamma[i] += noise[i];
+    for (int i = threadIdx.x; i < n * N; i += kAuxThreads) gamma[i] += noise[i];
+  }
+}
+
+// Batched form: workgroup b handles system b of a plan (forward systems only);
+// the noise is generated in place from (seed, b, i), so the launch reads nothing
+// but the plan's descriptors.
+template <typename T>
+__global__ void __launch_bounds__(kAuxThreads) batch_warm_start_kernel(const KSys *systems, uint64_t seed,
+                                                                       double scale) {
+  const KSys &d = systems[blockIdx.x];
+  if (!(d.flags & KF_FWD)) return;
+  const int n = (int)d.n, N = (int)d.N;
+  shift_knots(static_cast<T *>(d.lambda), n, N);
+  if (scale != 0.0) {
+    T *gamma = static_cast<T *>(const_cast<void *>(d.gamma));
+    const uint64_t key = counter_key(seed, blockIdx.x);
+    for (int i = threadIdx.x; i < n * N; i += kAuxThreads) gamma[i] += (T)(scale * counter_normal(key, (uint64_t)i));
   }
 }
 
@@ -289,6 +310,18 @@ hipError_t warm_start_launch(int dtype, uint32_t n, uint32_t N, void *lambda, vo
                        static_cast<float *>(lambda), static_cast<float *>(gamma), static_cast<const float *>(noise),
                        (int)n, (int)N);
   }
+  return hipGetLastError();
+}
+
+hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t count, uint64_t seed, double scale,
+                                   hipStream_t stream) {
+  if (count == 0) return hipSuccess;
+  if (dtype == GBDPCG_F64)
+    hipLaunchKernelGGL(batch_warm_start_kernel<double>, dim3(count), dim3(kAuxThreads), 0, stream, systems, seed,
+                       scale);
+  else
+    hipLaunchKernelGGL(batch_warm_start_kernel<float>, dim3(count), dim3(kAuxThreads), 0, stream, systems, seed,
+                       scale);
   return hipGetLastError();
 }
 

@@ -52,6 +52,11 @@ struct gbdpcg_batch {
   int nt = 256;      // threads per workgroup
 };
 
+namespace gbdpcg {
+bool batch_uses_registers(const gbdpcg_batch *plan) { return plan && plan->reg; }
+int batch_threads(const gbdpcg_batch *plan) { return plan ? plan->nt : 0; }
+}  // namespace gbdpcg
+
 namespace {
 
 constexpr int kXcds = 8;
@@ -139,10 +144,17 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   std::vector<uint32_t> G(count);
   std::vector<uint8_t> fused(count);
   size_t lds = 0;
+  // cfg->knots_per_wg caps the knots per workgroup (the autotuner's main knob)
+  const uint32_t kcap = cfg->knots_per_wg ? std::max<uint32_t>(cfg->knots_per_wg, 1u) : 0u;
   for (uint32_t i = 0; use_reg && i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
     const uint32_t kmax = reg_max_knots(d.n, nt);
-    uint32_t g = d.wgs ? d.wgs : (d.N + kmax - 1) / kmax;
+    const uint32_t target = kcap ? std::min(kmax, std::max<uint32_t>(kcap, 2u)) : kmax;
+    uint32_t g = d.wgs;
+    if (!g) {  // the cap is a target: every workgroup keeps at least 2 knots
+      g = (d.N + target - 1) / target;
+      g = std::min(g, std::max<uint32_t>(d.N / 2, 1u));
+    }
     if (g == 0 || (d.N + g - 1) / g > kmax || d.N / g < 2 || g > (uint32_t)kMaxWgsPerSystem) {
       set_error("system %u: %u workgroups do not suit the register kernel (N=%u, 2..%u knots each)", i, g, d.N,
                 kmax);
@@ -154,7 +166,8 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   }
   for (uint32_t i = 0; !use_reg && i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
-    const uint32_t kmax = max_knots(d.n, d.N, ts, budget);
+    uint32_t kmax = max_knots(d.n, d.N, ts, budget);
+    if (kcap && kmax) kmax = std::min(kmax, kcap);
     if (kmax == 0) {
       set_error("system %u (n=%u) does not fit %zu B of LDS even with one knot per workgroup", i, d.n, budget);
       return GBDPCG_ERR_TOO_LARGE;
@@ -387,6 +400,16 @@ int gbdpcg_batch_wgs(const gbdpcg_batch *plan, uint32_t *wgs_out) {
     return GBDPCG_ERR_INVALID;
   }
   std::copy(plan->wgs.begin(), plan->wgs.end(), wgs_out);
+  return GBDPCG_OK;
+}
+
+int gbdpcg_batch_warm_start(gbdpcg_batch *plan, uint64_t seed, double noise_scale, void *stream) {
+  if (!plan || !(noise_scale == noise_scale)) {
+    set_error("gbdpcg_batch_warm_start: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  HIPCHK(batch_warm_start_launch(plan->dtype, plan->d_sys, plan->count, seed, noise_scale,
+                                 static_cast<hipStream_t>(stream)));
   return GBDPCG_OK;
 }
 

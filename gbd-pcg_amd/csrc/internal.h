@@ -1,6 +1,7 @@
 // Internal definitions shared by the device kernels (kernels.hip) and the host
 // runtime (api.cpp).  Not part of the public ABI.
 #pragma once
+#include <math.h>
 #include <stddef.h>
 #include <stdint.h>
 
@@ -94,6 +95,25 @@ __host__ __device__ inline uint32_t make_item(uint32_t sys, uint32_t rank) {
   return (sys << 12) | rank;
 }
 
+// Counter-based standard normals shared by the host (gbdpcg_counter_randn) and
+// the batched warm-start kernel: splitmix64 (Steele, Lea, Flood 2014) as a hash of
+// (key + j), two hashes per value, Box-Muller in double.
+__host__ __device__ inline uint64_t smix64(uint64_t x) {
+  uint64_t z = x + 0x9e3779b97f4a7c15ULL;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline uint64_t counter_key(uint64_t seed, uint64_t stream) {
+  return smix64(seed) ^ (stream * 0xd1b54a32d192ed03ULL);
+}
+__host__ __device__ inline double counter_normal(uint64_t key, uint64_t i) {
+  const uint64_t h1 = smix64(key + 2 * i), h2 = smix64(key + 2 * i + 1);
+  const double u1 = (double)((h1 >> 11) + 1) * 0x1.0p-53;  // (0, 1]
+  const double u2 = (double)(h2 >> 11) * 0x1.0p-53;        // [0, 1)
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
 }  // namespace gbdpcg
 
 // ---- host-side helpers implemented next to the kernels ----
@@ -112,6 +132,8 @@ hipError_t schur_launch(int dtype, uint32_t n, uint32_t m, uint32_t N, const voi
                         const void *Qd, const void *Rd, void *S, hipStream_t stream);
 hipError_t warm_start_launch(int dtype, uint32_t n, uint32_t N, void *lambda, void *gamma, const void *noise,
                              hipStream_t stream);
+hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t count, uint64_t seed, double scale,
+                                   hipStream_t stream);
 // pcg_reg_kernel.hip (register-resident latency kernel, fused recurrence)
 bool reg_supported(int n);
 size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt);
@@ -120,4 +142,21 @@ hipError_t reg_prepare(int dtype, int n, int nt, size_t lds);
 hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu);
 hipError_t reg_launch(int dtype, int n, int nt, const KSys &one, const KSys *systems, const uint32_t *items,
                       uint32_t rounds, uint32_t stride, uint32_t grid, size_t lds, hipStream_t stream);
+// tune.cpp: autotuner choice cache (a..d: meaning fixed by the key's kind)
+struct TuneChoice {
+  uint32_t a, b, c, d;
+};
+}  // namespace gbdpcg
+#include <string>
+namespace gbdpcg {
+std::string device_tag(int dev);
+bool tune_lookup(const std::string &key, TuneChoice *out);
+void tune_store(const std::string &key, const TuneChoice &c);
+void tune_forget_all();
+}  // namespace gbdpcg
+struct gbdpcg_batch;
+namespace gbdpcg {
+// batch.cpp: properties of a built plan used by the tuner
+bool batch_uses_registers(const gbdpcg_batch *plan);
+int batch_threads(const gbdpcg_batch *plan);
 }  // namespace gbdpcg

@@ -124,6 +124,17 @@ extern "C" int gbdpcg_randn(int dtype, uint64_t seed, uint64_t count, double sca
   return GBDPCG_OK;
 }
 
+extern "C" int gbdpcg_counter_randn(int dtype, uint64_t seed, uint64_t stream, uint64_t count, double scale,
+                                    void *h_out) {
+  if ((dtype != GBDPCG_F32 && dtype != GBDPCG_F64) || (count && !h_out)) {
+    set_error("gbdpcg_counter_randn: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  const uint64_t key = counter_key(seed, stream);
+  for (uint64_t i = 0; i < count; ++i) put(dtype, h_out, i, scale * counter_normal(key, i));
+  return GBDPCG_OK;
+}
+
 extern "C" int gbdpcg_lqr_schur_host(int dtype, uint32_t n, uint32_t m, uint32_t N, const double *A,
                                      const double *B, const double *Qd, const double *Rd, void *h_S) {
   if ((dtype != GBDPCG_F32 && dtype != GBDPCG_F64) || n == 0 || N == 0 || !A || (m && !B) || !Qd ||

@@ -46,7 +46,7 @@ class Config(ctypes.Structure):
     _fields_ = [("exit_tol", ctypes.c_double), ("max_iter", ctypes.c_uint32), ("grid", ctypes.c_uint32 * 3),
                 ("block", ctypes.c_uint32 * 3), ("empty_pinv", ctypes.c_int32), ("exit_mode", ctypes.c_int32),
                 ("precond", ctypes.c_int32), ("recurrence", ctypes.c_int32), ("stream", ctypes.c_void_p),
-                ("kernel", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 7)]
+                ("kernel", ctypes.c_uint32), ("knots_per_wg", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
 
 
 class Result(ctypes.Structure):
@@ -70,6 +70,15 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("grid", ctypes.c_uint32), ("rounds", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
                 ("lds_bytes", ctypes.c_uint32), ("systems", ctypes.c_uint32), ("total_items", ctypes.c_uint32),
                 ("est_cost", ctypes.c_double)]
+
+
+class TuneReport(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_uint32), ("from_cache", ctypes.c_uint32), ("knots_per_wg", ctypes.c_uint32),
+                ("threads", ctypes.c_uint32), ("wgs", ctypes.c_uint32), ("kernel", ctypes.c_uint32),
+                ("best_us", ctypes.c_double), ("default_us", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 RESULT_BYTES = ctypes.sizeof(Result)
@@ -105,12 +114,17 @@ SIGNATURES = [
     ("gbdpcg_batch_info_get", _i32, [_vp, ctypes.POINTER(BatchInfo)]),
     ("gbdpcg_batch_wgs", _i32, [_vp, _PU]),
     ("gbdpcg_batch_destroy", _i32, [_vp]),
+    ("gbdpcg_batch_warm_start", _i32, [_vp, _u64, _dbl, _vp]),
+    ("gbdpcg_tune_single", _i32, [_i32, _u32, _u32, _vp, _vp, _vp, _PC, _u32, ctypes.POINTER(TuneReport)]),
+    ("gbdpcg_batch_autotune", _i32, [ctypes.POINTER(_vp), _i32, ctypes.POINTER(SystemDesc), _u32, _PC, _u32,
+                                     ctypes.POINTER(TuneReport)]),
     ("gbdpcg_precond_device", _i32, [_i32, _u32, _u32, _vp, _vp, _i32, _vp]),
     ("gbdpcg_precond_host", _i32, [_i32, _u32, _u32, _vp, _vp, _i32]),
     ("gbdpcg_lqr_generate", _i32, [_i32, _u32, _u32, _u32, _u64, _dbl, _vp, _vp, _PD, _PD, _PD, _PD]),
     ("gbdpcg_lqr_schur_host", _i32, [_i32, _u32, _u32, _u32, _PD, _PD, _PD, _PD, _vp]),
     ("gbdpcg_lqr_schur_device", _i32, [_i32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("gbdpcg_randn", _i32, [_i32, _u64, _u64, _dbl, _vp]),
+    ("gbdpcg_counter_randn", _i32, [_i32, _u64, _u64, _u64, _dbl, _vp]),
     ("gbdpcg_warm_start_device", _i32, [_i32, _u32, _u32, _vp, _vp, _vp, _vp]),
 ]
 
@@ -157,7 +171,8 @@ def _dtype_code(dt) -> int:
 
 def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS_ETA,
                 recurrence: int = RECUR_REFERENCE, precond: int = PRECOND_SS, wgs: int = 0,
-                stream: Optional[int] = None, kernel: int = KERNEL_AUTO) -> Config:
+                stream: Optional[int] = None, kernel: int = KERNEL_AUTO, knots_per_wg: int = 0,
+                threads: int = 0) -> Config:
     cfg = Config()
     lib().gbdpcg_config_init(ctypes.byref(cfg))
     cfg.exit_tol = float(tol)
@@ -168,6 +183,9 @@ def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS
     cfg.grid[0] = int(wgs)
     cfg.stream = stream
     cfg.kernel = int(kernel)
+    cfg.knots_per_wg = int(knots_per_wg)
+    if threads:
+        cfg.block[0] = int(threads)
     return cfg
 
 
@@ -330,6 +348,24 @@ def randn(count: int, seed: int, scale: float = 1.0, dtype=np.float32) -> np.nda
     return out
 
 
+def counter_randn(count: int, seed: int, stream: int = 0, scale: float = 1.0, dtype=np.float32) -> np.ndarray:
+    """The counter-based normals the batched warm start adds (same values on the host)."""
+    out = np.zeros(count, dtype=dtype)
+    check(lib().gbdpcg_counter_randn(_dtype_code(np.dtype(dtype)), seed, stream, count, scale, _np_ptr(out)))
+    return out
+
+
+def tune_single(S, gamma, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, exit_mode=EXIT_ABS_ETA,
+                recurrence=RECUR_REFERENCE, precond=PRECOND_SS, kernel=KERNEL_AUTO, trials=5, stream=None) -> dict:
+    """Times the candidate layouts of (n, N) on these inputs and makes the fastest
+    the default for later automatic solves of this shape.  Returns the report."""
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, 0, _stream_handle(stream), kernel)
+    rep = TuneReport()
+    check(lib().gbdpcg_tune_single(_dtype_code(S.dtype), n, N, _dev(S, "S"), _dev(Pinv, "Pinv"),
+                                   _dev(gamma, "gamma"), ctypes.byref(cfg), int(trials), ctypes.byref(rep)))
+    return rep.as_dict()
+
+
 def lds_bytes(dtype, n: int, knots_per_wg: int, wgs: int) -> int:
     return lib().gbdpcg_lds_bytes(_dtype_code(dtype), n, knots_per_wg, wgs)
 
@@ -344,7 +380,8 @@ class Batch:
     forward (default True), adjoint (default False), wgs."""
 
     def __init__(self, systems: Sequence[dict], *, exit_mode=EXIT_REL_RESID, recurrence=RECUR_REFERENCE,
-                 lds_budget: int = 0, tol=1e-5, max_iter=173):
+                 lds_budget: int = 0, tol=1e-5, max_iter=173, knots_per_wg: int = 0, threads: int = 0,
+                 kernel: int = KERNEL_AUTO, autotune: bool = False, trials: int = 3, stream=None):
         if not systems:
             raise ValueError("empty batch")
         self._keep = list(systems)
@@ -361,14 +398,28 @@ class Batch:
             d.max_iter = int(s.get("max_iter", max_iter))
             d.flags = (SYS_FORWARD if s.get("forward", True) else 0) | (SYS_ADJOINT if s.get("adjoint", False) else 0)
             d.wgs = int(s.get("wgs", 0))
-        cfg = make_config(exit_mode=exit_mode, recurrence=recurrence)
+        cfg = make_config(exit_mode=exit_mode, recurrence=recurrence, stream=_stream_handle(stream), kernel=kernel,
+                          knots_per_wg=knots_per_wg, threads=threads)
         h = ctypes.c_void_p()
-        check(lib().gbdpcg_batch_create(ctypes.byref(h), dt, arr, len(systems), ctypes.byref(cfg), int(lds_budget)))
+        self.tune_report = None
+        if autotune:
+            rep = TuneReport()
+            check(lib().gbdpcg_batch_autotune(ctypes.byref(h), dt, arr, len(systems), ctypes.byref(cfg), int(trials),
+                                              ctypes.byref(rep)))
+            self.tune_report = rep.as_dict()
+        else:
+            check(lib().gbdpcg_batch_create(ctypes.byref(h), dt, arr, len(systems), ctypes.byref(cfg),
+                                            int(lds_budget)))
         self._h = h
         self.count = len(systems)
 
     def run(self, stream=None) -> None:
         check(lib().gbdpcg_batch_run(self._h, _stream_handle(stream)))
+
+    def warm_start(self, seed: int, noise_scale: float = 0.0, stream=None) -> None:
+        """lambda shifted one knot towards the start, gamma += noise_scale * counter
+        normals (seed, system index) for every forward system, in one launch."""
+        check(lib().gbdpcg_batch_warm_start(self._h, int(seed), float(noise_scale), _stream_handle(stream)))
 
     def results(self, stream=None):
         res = (Result * self.count)()

@@ -89,7 +89,8 @@ typedef struct gbdpcg_config {
   int32_t recurrence; /* GBDPCG_RECUR_*, default REFERENCE                               */
   void *stream;       /* hipStream_t; NULL = the default stream                          */
   uint32_t kernel;    /* GBDPCG_KERNEL_*: 0 = automatic                                  */
-  uint32_t reserved[7];
+  uint32_t knots_per_wg; /* cap on knots per workgroup, 0 = automatic (tuned or planner) */
+  uint32_t reserved[6];
 } gbdpcg_config;
 
 /* kernel selection (gbdpcg_config.kernel) */
@@ -253,6 +254,48 @@ int gbdpcg_batch_info_get(const gbdpcg_batch *plan, gbdpcg_batch_info *info);
 int gbdpcg_batch_wgs(const gbdpcg_batch *plan, uint32_t *wgs_out);
 int gbdpcg_batch_destroy(gbdpcg_batch *plan);
 
+/* Receding-horizon warm start of every system of a plan, in one launch:
+ * lambda[k] <- lambda[k+1] for k < N-1 (lambda[N-1] kept), then, if noise_scale != 0,
+ * gamma[i] += noise_scale * z_i with z_i = gbdpcg_counter_randn(seed, system index, i).
+ * gamma is modified in place (the descriptor's gamma pointer must be writable).
+ * Launch only (no allocation, no memset), so it can be captured with the run. */
+int gbdpcg_batch_warm_start(gbdpcg_batch *plan, uint64_t seed, double noise_scale, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Autotuner — extension.  Times candidate layouts (threads per workgroup, knots
+ * per workgroup, XCD placement, kernel) on the caller's own data and keeps the
+ * fastest.  Choices are cached per process, keyed by (device, dtype, shapes,
+ * recurrence, exit mode, flags); if the environment variable GBDPCG_TUNE_CACHE
+ * names a file, choices are also read from and appended to it.
+ * ------------------------------------------------------------------------- */
+typedef struct gbdpcg_tune_report {
+  uint32_t candidates;     /* layouts timed (0 when the choice came from the cache)    */
+  uint32_t from_cache;     /* 1: cached choice, nothing was timed                      */
+  uint32_t knots_per_wg;   /* chosen cap on knots per workgroup                        */
+  uint32_t threads;        /* chosen threads per workgroup                             */
+  uint32_t wgs;            /* single solve: chosen workgroups; batch: launch grid      */
+  uint32_t kernel;         /* GBDPCG_KERNEL_LDS or GBDPCG_KERNEL_REGISTER              */
+  double best_us;          /* mean time of one solve / one batch run, chosen layout    */
+  double default_us;       /* same for the untuned default layout                      */
+} gbdpcg_tune_report;
+
+/* Single solve: times the candidate layouts of (n, N) with cfg's recurrence and exit
+ * mode on d_S / d_Pinv / d_gamma (device pointers; lambda0 = 0 in a scratch buffer,
+ * the caller's data is not modified) and makes the fastest the default for later
+ * solves of this shape on this device whose cfg leaves grid[0], knots_per_wg and
+ * kernel automatic.  trials = timed runs per candidate (0 = 5). */
+int gbdpcg_tune_single(int dtype, uint32_t stateSize, uint32_t knotPoints, const void *d_S, const void *d_Pinv,
+                       const void *d_gamma, const gbdpcg_config *cfg, uint32_t trials,
+                       gbdpcg_tune_report *report);
+
+/* Batch: builds candidate plans of the same systems (cfg->knots_per_wg and
+ * cfg->block[0] varied), runs each on the systems' own data and returns the
+ * fastest plan in *out.  The forward systems' lambda is restored to its value at
+ * entry; adjoint outputs hold the last trial's values.  Not allowed during graph
+ * capture.  trials = timed runs per candidate (0 = 3). */
+int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc *systems, uint32_t count,
+                          const gbdpcg_config *cfg, uint32_t trials, gbdpcg_tune_report *report);
+
 /* ---------------------------------------------------------------------------
  * Preconditioner formation (device) — extension (the documented pcg_solve has no
  * Pinv argument, README.md:42, and empty_pinv = 1 is the default, types.cuh:27).
@@ -288,6 +331,10 @@ int gbdpcg_lqr_schur_device(int dtype, uint32_t n, uint32_t m, uint32_t N, const
                             void *stream);
 /* Standard-normal vector from the same generator (count values, scale applied). */
 int gbdpcg_randn(int dtype, uint64_t seed, uint64_t count, double scale, void *h_out);
+/* Counter-based normals (host), the values gbdpcg_batch_warm_start adds on the device:
+ * z_i = sqrt(-2 ln u1) cos(2 pi u2), u1 = (h(2i) >> 11 + 1) 2^-53, u2 = (h(2i+1) >> 11) 2^-53,
+ * h(j) = splitmix64(splitmix64(seed) ^ (stream * 0xD1B54A32D192ED03) + j). */
+int gbdpcg_counter_randn(int dtype, uint64_t seed, uint64_t stream, uint64_t count, double scale, void *h_out);
 
 /* Warm start for receding-horizon re-solves (device): lambda[k] <- lambda[k+1] for
  * k < N-1 (lambda[N-1] kept), gamma += noise (noise may be NULL). */

@@ -65,6 +65,7 @@ struct pcg_config {
   int recurrence = GBDPCG_RECUR_REFERENCE;
   void *stream = nullptr;
   uint32_t kernel = GBDPCG_KERNEL_AUTO;
+  uint32_t knots_per_wg = 0;    // 0 = automatic (tuned by gbdpcg_tune_single, else the planner)
   gbdpcg_result last_result{};  // filled by the device-pointer solve
 
   pcg_config(T exit_tol = pcg_constants::DEFAULT_EPSILON<T>,
@@ -100,6 +101,7 @@ inline gbdpcg_config to_c(const pcg_config<T> *cfg) {
     c.recurrence = cfg->recurrence;
     c.stream = cfg->stream;
     c.kernel = cfg->kernel;
+    c.knots_per_wg = cfg->knots_per_wg;
   }
   return c;
 }
