@@ -16,7 +16,13 @@ Also reported on the same line:
   * cpu_baseline: the FP64 host restatement of the reference iteration (oracle/,
     single thread) on a time-bounded sample of the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+  * uneven_512: 512 systems, n from {6, 12, 14, 28}, N uniform in [16, 256],
+    forward solves at tol 1e-5 in one autotuned, graph-replayed launch;
+  * mpc_sweep_n28_N512: 256 systems n=28, N=512, forward + adjoint, ten
+    warm-started re-solves (lambda shifted one knot, gamma += 1e-2 N(0,1)) in one
+    graph, FP32 and FP64.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-batched] [--no-workloads]
 """
 import argparse
 import json
@@ -39,6 +45,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true", help="skip the 512-system and n=28/N=512 workloads")
+    ap.add_argument("--no-tune", action="store_true", help="time the planner's default layout, not the tuned one")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -62,6 +70,9 @@ def main():
     P = torch.empty_like(S)
     gbdpcg.precond_device(S, P, n, N, gbdpcg.PRECOND_SS)
     kw = dict(tol=tol, max_iter=max_iter, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+
+    # the autotuner picks the layout of this shape (timed on these inputs)
+    tune = None if args.no_tune else gbdpcg.tune_single(S, g, n, N, P, **kw)
 
     # one zeroed lambda per step so every step is a full solve from lambda0 = 0
     K, W = args.steps, args.warmup
@@ -151,7 +162,8 @@ def main():
                                 lam=torch.zeros(n * N, device=dev), n=n, N=N, tol=tol, max_iter=max_iter,
                                 glam=torch.from_numpy(gbdpcg.randn(n * N, 7000 + b)).to(dev),
                                 ggamma=torch.zeros(n * N, device=dev), gS=torch.zeros_like(Sb), adjoint=True))
-        plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+        plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
+                            autotune=not args.no_tune)
         plan.run()
         bres = plan.results()
         graph = torch.cuda.CUDAGraph()
@@ -177,7 +189,13 @@ def main():
                    "mean_iters_fwd": float(np.mean([r.iters for r in bres])),
                    "mean_iters_adj": float(np.mean([r.adj_iters for r in bres])),
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
-                   "plan": plan.info()}
+                   "plan": plan.info(), "tune": plan.tune_report}
+
+    workloads = None
+    if not args.no_workloads:
+        workloads = {"uneven_512": uneven_512(gbdpcg, torch, np, dev, not args.no_tune)}
+        for dt in (np.float32, np.float64):
+            workloads[f"mpc_sweep_n28_N512_{np.dtype(dt).name}"] = mpc_sweep(gbdpcg, torch, np, dev, dt)
 
     if args.calibrate_copy:
         # a plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch give the
@@ -216,9 +234,94 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
         "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
-        "reference_recurrence_us": round(ref_us, 2), "batched": batched,
+        "reference_recurrence_us": round(ref_us, 2), "tune": tune, "batched": batched, "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
+
+
+def _generate(gbdpcg, np, specs, seed0, dtype):
+    """Seeded LQR Schur systems, generated on host threads (the generator releases
+    the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(lambda a: gbdpcg.lqr_generate(a[1][0], a[1][0] // 2, a[1][1], seed0 + a[0], dtype=dtype),
+                           enumerate(specs)))
+
+
+def _graph_time(torch, body, replays):
+    """Captures body(stream) in one graph, replays it once untimed, then times
+    `replays` replays with events on the replay stream."""
+    stream = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=stream):
+        body(stream)
+    graph.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    cur = torch.cuda.current_stream()
+    a.record(cur)
+    for _ in range(replays):
+        graph.replay()
+    b.record(cur)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) * 1000.0 / replays
+
+
+def uneven_512(gbdpcg, torch, np, dev, autotune):
+    rng = np.random.default_rng(3000)
+    specs = [(int(rng.choice([6, 12, 14, 28])), int(rng.integers(16, 257))) for _ in range(512)]
+    systems = []
+    for (n, N), (S_h, g_h) in zip(specs, _generate(gbdpcg, np, specs, 3000, np.float32)):
+        S = torch.from_numpy(S_h).to(dev)
+        P = torch.empty_like(S)
+        gbdpcg.precond_device(S, P, n, N, gbdpcg.PRECOND_SS)
+        systems.append(dict(S=S, Pinv=P, gamma=torch.from_numpy(g_h).to(dev), lam=torch.zeros(n * N, device=dev),
+                            n=n, N=N, tol=1e-5, max_iter=173))
+    plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, autotune=autotune)
+    lams = [d["lam"] for d in systems]
+
+    def body(st):
+        torch._foreach_zero_(lams)
+        plan.run(stream=st)
+
+    us = _graph_time(torch, body, 10)
+    res = plan.results()
+    return {"systems": 512, "us_per_batch": round(us, 1), "solves_per_s": round(512 / (us * 1e-6), 1),
+            "mean_iters": float(np.mean([r.iters for r in res])),
+            "all_converged": all(r.status == 0 for r in res), "plan": plan.info(), "tune": plan.tune_report}
+
+
+def mpc_sweep(gbdpcg, torch, np, dev, dtype, count=256, n=28, N=512, steps=10):
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    systems = []
+    for i, (S_h, g_h) in enumerate(_generate(gbdpcg, np, [(n, N)] * count, 5100, dtype)):
+        S = torch.from_numpy(S_h).to(dev)
+        P = torch.empty_like(S)
+        gbdpcg.precond_device(S, P, n, N, gbdpcg.PRECOND_SS)
+        systems.append(dict(S=S, Pinv=P, gamma=torch.from_numpy(g_h).to(dev), lam=torch.zeros(n * N, dtype=tdt,
+                            device=dev), n=n, N=N, tol=1e-5, max_iter=173, adjoint=True,
+                            glam=torch.from_numpy(gbdpcg.randn(n * N, 9100 + i, dtype=dtype)).to(dev),
+                            ggamma=torch.zeros(n * N, dtype=tdt, device=dev), gS=torch.zeros_like(S)))
+        del S_h, g_h
+    plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+
+    def body(st):
+        for k in range(steps):
+            plan.warm_start(seed=900 + k, noise_scale=1e-2, stream=st)
+            plan.run(stream=st)
+
+    us = _graph_time(torch, body, 2)
+    res = plan.results()
+    out = {"systems": count, "re_solves": steps, "ms_per_graph": round(us / 1000.0, 2),
+           "fwd_adj_solves_per_s": round(count * steps / (us * 1e-6), 1),
+           "last_mean_iters_fwd": float(np.mean([r.iters for r in res])),
+           "last_mean_iters_adj": float(np.mean([r.adj_iters for r in res])),
+           "all_converged": all(r.status == 0 and r.adj_status == 0 for r in res), "plan": plan.info()}
+    plan.close()
+    del systems
+    torch.cuda.empty_cache()
+    return out
 
 
 if __name__ == "__main__":

@@ -24,6 +24,14 @@
 namespace gbdpcg {
 
 namespace {
+// threadIdx.x through an empty asm: index arithmetic derived from it is then
+// recomputed where it is used instead of being hoisted to the kernel entry and
+// kept live (in VGPRs) across the whole solve.
+__device__ __forceinline__ int opaque_tid() {
+  int t = (int)threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
 
 struct RegOut {
   int iters, status;
@@ -67,18 +75,20 @@ struct RegItem {
     const int kl = m - 1;
     const uint32_t epoch = base + seq;
     uint64_t *r = rec((int)(seq & 1u), rank);
+    int rw = row;  // recomputed per call: the slot addresses need not stay live
+    asm volatile("" : "+v"(rw));
 #pragma unroll
     for (int e = 0; e < 2; ++e) {
       if (e >= edge) break;
       if (kl == e) {
 #pragma unroll
         for (int qq = 0; qq < WPV; ++qq)
-          xstore(r + (kRecScalars + e * n + row) * WPV + qq, payload_word(val, qq), epoch);
+          xstore(r + (kRecScalars + e * n + rw) * WPV + qq, payload_word(val, qq), epoch);
       }
       if (kl == K - 1 - e) {
 #pragma unroll
         for (int qq = 0; qq < WPV; ++qq)
-          xstore(r + (kRecScalars + 2 * n + e * n + row) * WPV + qq, payload_word(val, qq), epoch);
+          xstore(r + (kRecScalars + 2 * n + e * n + rw) * WPV + qq, payload_word(val, qq), epoch);
       }
     }
   }
@@ -145,7 +155,7 @@ struct RegItem {
     const uint32_t epoch = base + seq;
     const int type = (int)(seq & 1u);
     bool ok = true;
-    const int t = threadIdx.x;
+    const int t = opaque_tid();
     if (t < 64) {
       constexpr int NW = 3 * WPV;
       T acc0 = 0, acc1 = 0, acc2 = 0;
@@ -245,18 +255,23 @@ template <typename T, int NS, int CW>
 __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A, int kg, int row, int c, int N,
                                            bool active) {
   constexpr int n = NS, nP = (int)padded_n(NS);
+  // opaque copies: keeps the per-lane offsets below from being hoisted out of the
+  // batched item loop (they would stay live, in registers, for the whole solve)
+  int cl = c, rl = row;
+  asm volatile("" : "+v"(cl), "+v"(rl));
 #pragma unroll
   for (int j = 0; j < CW; ++j) {
-    const int col = c * CW + j;
+    const int col = cl * CW + j;
     const int seg = col / nP, cc = col - seg * nP;
     T v = 0;
     if (active && kg >= 0 && kg < N && seg < 3 && cc < n && !(kg == 0 && seg == 0) && !(kg == N - 1 && seg == 2))
-      v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)(seg * n + cc) * n + row];
+      v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)(seg * n + cc) * n + rl];
     dst[j] = v;
   }
 }
 
 __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
+
 
 template <typename T, int NS, int NT>
 __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
@@ -312,7 +327,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
     it.ctl[0] = 0;
     it.ctl[1] = (int)base;
   }
-  for (int i = t; i < total; i += NT) lds[i] = (T)0;
+  for (int i = opaque_tid(); i < total; i += NT) lds[i] = (T)0;
 
   // matrix chunks straight from HBM into registers
   T mM[CW], mS[CW];
@@ -328,7 +343,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   {
     const T *lam0 = static_cast<const T *>(d.lambda);
     T *dst = fwd ? it.xv : it.lv;
-    for (int i = t; i < (K + 2) * n; i += NT) {
+    for (int i = opaque_tid(); i < (K + 2) * n; i += NT) {
       const int jj = i / n, e = i - jj * n;
       const int k = it.a - 1 + jj;
       if (k >= 0 && k < it.N) dst[(jj + 1) * nP + e] = gptr(lam0)[(size_t)k * n + e];
@@ -344,9 +359,9 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       // adjoint: lambda (own + ghosts) was left in xv by the forward pass, or
       // loaded into lv above; keep it in lv and restart from x = 0
       if (fwd) {
-        for (int i = t; i < it.VL; i += NT) it.lv[i] = it.xv[i];
+        for (int i = opaque_tid(); i < it.VL; i += NT) it.lv[i] = it.xv[i];
       }
-      for (int i = t; i < it.VL; i += NT) it.xv[i] = (T)0;
+      for (int i = opaque_tid(); i < it.VL; i += NT) it.xv[i] = (T)0;
       __syncthreads();
     }
     const T *rhs = static_cast<const T *>(pass == 0 ? d.gamma : d.glambda);
@@ -502,7 +517,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       // g_S block rows of the own knots: -1/2 (gg_k lam_j^T + lam_k gg_j^T), j in {k-1,k,k+1}
       const int bs = 3 * n * n;
       T *gS = static_cast<T *>(d.gS);
-      for (int e = t; e < K * bs; e += NT) {
+      for (int e = opaque_tid(); e < K * bs; e += NT) {
         const int kl = e / bs;
         const int wq = e - kl * bs;
         const int col = wq / n;
