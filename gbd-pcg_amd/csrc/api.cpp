@@ -12,6 +12,7 @@
 // include/interface.cuh:112); device buffers are cached, not allocated per call.
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -51,6 +52,14 @@ constexpr uint32_t kMaxIterLimit = 100000;  // keeps a solve's epochs far from w
 constexpr int kXcds = 8;                    // gfx950: 8 XCDs, workgroups dealt round-robin
 
 static size_t tsize(int dtype) { return dtype == GBDPCG_F64 ? 8 : 4; }
+
+uint32_t exchange_flags() {
+  static const uint32_t f = [] {
+    const char *e = getenv("GBDPCG_DEVICE_SCOPE_EXCHANGE");
+    return (e && *e && strcmp(e, "0") != 0) ? (uint32_t)KF_NOXCD : 0u;
+  }();
+  return f;
+}
 static bool valid_dtype(int dtype) { return dtype == GBDPCG_F32 || dtype == GBDPCG_F64; }
 
 // ---------------------------------------------------------------------------
@@ -456,7 +465,8 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
   one.N = N;
   one.G = L.G;
   one.max_iter = cfg->max_iter;
-  one.flags = flags | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u);
+  one.flags = flags | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
+              exchange_flags();
   if (L.reg) {
     HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
   } else {
@@ -740,7 +750,8 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     one.N = N;
     one.G = L.G;
     one.max_iter = cfg->max_iter;
-    one.flags = KF_FWD | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u);
+    one.flags = KF_FWD | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
+                exchange_flags();
     double total = 0.0;
     for (uint32_t t = 0; t <= trials; ++t) {
       if ((e = hipMemsetAsync(lam, 0, (size_t)n * N * ts, st)) != hipSuccess) return fail("hipMemsetAsync", e);

@@ -314,7 +314,8 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     k.G = G[i];
     k.max_iter = d.max_iter;
     k.flags = ((d.flags & GBDPCG_SYS_FORWARD) ? KF_FWD : 0u) | ((d.flags & GBDPCG_SYS_ADJOINT) ? KF_ADJ : 0u) |
-              (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused[i] ? KF_FUSED : 0u);
+              (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused[i] ? KF_FUSED : 0u) |
+              exchange_flags();
   }
   if ((e = hipMemset(plan->d_slots, 0, words * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMemset(plan->d_results, 0, count * sizeof(gbdpcg_result))) != hipSuccess ||

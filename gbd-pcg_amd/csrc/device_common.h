@@ -30,6 +30,39 @@ __device__ __forceinline__ void xstore(uint64_t *p, uint32_t payload, uint32_t e
   __hip_atomic_store(gptr(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Same word with a workgroup-scope store: it completes in the XCD's shared L2
+// (the CU's L1 is write-through) instead of being written through to memory, and
+// the device-scope polls below, which skip the L1, find it there.  Only valid
+// when the writer and every reader of the word run on the same XCD; the kernels
+// check that at run time (xcc_id) before using it.
+__device__ __forceinline__ void xstore_xcd(uint64_t *p, uint32_t payload, uint32_t epoch) {
+  const uint64_t w = ((uint64_t)epoch << 32) | (uint64_t)payload;
+  __hip_atomic_store(gptr(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void xstore_sel(uint64_t *p, uint32_t payload, uint32_t epoch, bool xcd) {
+  if (xcd)
+    xstore_xcd(p, payload, epoch);
+  else
+    xstore(p, payload, epoch);
+}
+
+// XCD (accelerator complex die) this wave runs on: hardware register XCC_ID,
+// bits [3:0].
+__device__ __forceinline__ uint32_t xcc_id() { return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xFu; }
+
+// A value every lane holds identically, moved to a scalar register: later
+// arithmetic on it can use SGPR operands and branches on it become scalar.
+__device__ __forceinline__ float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 __device__ __forceinline__ uint64_t xload(uint64_t *p) {
   return __hip_atomic_load(gptr(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -85,15 +118,14 @@ __device__ __forceinline__ bool xwait_words(uint64_t *p, uint32_t epoch, uint32_
   return false;
 }
 
-// a / b for the scalar recurrences.  FP32: reciprocal, one Newton step and a
-// residual correction (no denormal-mode switches, unlike the IEEE sequence);
-// b must be finite and non-zero, which the callers check first.
+// a / b for the scalar recurrences.  FP32: hardware reciprocal (1 ulp) and one
+// residual correction of the quotient, about 1 ulp in all (no denormal-mode
+// switches, unlike the IEEE sequence, and four dependent operations instead of
+// ten); b must be finite and non-zero, which the callers check first.
 template <typename T>
 __device__ __forceinline__ T qdiv(T a, T b) {
   if constexpr (sizeof(T) == 4) {
-    float r = __builtin_amdgcn_rcpf(b);
-    const float e = __builtin_fmaf(-b, r, 1.0f);
-    r = __builtin_fmaf(r, e, r);
+    const float r = __builtin_amdgcn_rcpf(b);
     const float q = a * r;
     const float res = __builtin_fmaf(-b, q, a);
     return __builtin_fmaf(res, r, q);
@@ -161,6 +193,33 @@ __device__ __forceinline__ T wave_sum(T v) {
   v += dpp<kDppRowBcast15, 0xa>(v);
   v += dpp<kDppRowBcast31, 0xc>(v);
   return read_lane63(v);
+}
+
+// Three independent wave sums with their DPP steps interleaved (a DPP read of a
+// value written by the previous VALU op must wait; three chains hide that).
+template <typename T>
+__device__ __forceinline__ void wave_sum3(T &a, T &b, T &c) {
+  a += dpp<kDppRowShr1>(a);
+  b += dpp<kDppRowShr1>(b);
+  c += dpp<kDppRowShr1>(c);
+  a += dpp<kDppRowShr2>(a);
+  b += dpp<kDppRowShr2>(b);
+  c += dpp<kDppRowShr2>(c);
+  a += dpp<kDppRowShr4>(a);
+  b += dpp<kDppRowShr4>(b);
+  c += dpp<kDppRowShr4>(c);
+  a += dpp<kDppRowShr8>(a);
+  b += dpp<kDppRowShr8>(b);
+  c += dpp<kDppRowShr8>(c);
+  a += dpp<kDppRowBcast15, 0xa>(a);
+  b += dpp<kDppRowBcast15, 0xa>(b);
+  c += dpp<kDppRowBcast15, 0xa>(c);
+  a += dpp<kDppRowBcast31, 0xc>(a);
+  b += dpp<kDppRowBcast31, 0xc>(b);
+  c += dpp<kDppRowBcast31, 0xc>(c);
+  a = read_lane63(a);
+  b = read_lane63(b);
+  c = read_lane63(c);
 }
 
 // Sum over each aligned group of 4 lanes; all 4 lanes get the same bits.

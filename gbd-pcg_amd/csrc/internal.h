@@ -25,6 +25,7 @@ enum : uint32_t {
   KF_ADJ = 2u,    // adjoint solve S g_gamma = g_lambda (+ g_S)
   KF_REL = 4u,    // relative-residual exit test instead of |r'z| < tol
   KF_FUSED = 8u,  // single-reduction (Chronopoulos-Gear) recurrence
+  KF_NOXCD = 16u, // never use XCD-local (workgroup-scope) exchange stores
 };
 
 // Device-side descriptor of one system (one PCG instance).
@@ -156,6 +157,9 @@ void tune_forget_all();
 }  // namespace gbdpcg
 struct gbdpcg_batch;
 namespace gbdpcg {
+// api.cpp: KF_NOXCD when the environment variable GBDPCG_DEVICE_SCOPE_EXCHANGE is
+// set to a non-empty value other than "0" (read once), else 0
+uint32_t exchange_flags();
 // batch.cpp: properties of a built plan used by the tuner
 bool batch_uses_registers(const gbdpcg_batch *plan);
 int batch_threads(const gbdpcg_batch *plan);

@@ -55,6 +55,8 @@ struct RegItem {
   uint32_t base, seq;
   double tol;
   bool rel;
+  bool xcd;     // every rank on this workgroup's XCD: exchange stores stay in L2
+  bool xcd_ok;  // the system allows XCD-local stores (KF_NOXCD clear)
   // lane roles
   int q, c, m, row, kg;
   bool act_u, act_w, owner, ghost;
@@ -83,23 +85,24 @@ struct RegItem {
       if (kl == e) {
 #pragma unroll
         for (int qq = 0; qq < WPV; ++qq)
-          xstore(r + (kRecScalars + e * n + rw) * WPV + qq, payload_word(val, qq), epoch);
+          xstore_sel(r + (kRecScalars + e * n + rw) * WPV + qq, payload_word(val, qq), epoch, xcd);
       }
       if (kl == K - 1 - e) {
 #pragma unroll
         for (int qq = 0; qq < WPV; ++qq)
-          xstore(r + (kRecScalars + 2 * n + e * n + rw) * WPV + qq, payload_word(val, qq), epoch);
+          xstore_sel(r + (kRecScalars + 2 * n + e * n + rw) * WPV + qq, payload_word(val, qq), epoch, xcd);
       }
     }
   }
 
   // Workgroup totals of three per-lane partials (DPP per wave, then the waves in
   // order by thread 0), published as this rank's three scalars; with one
-  // workgroup they go straight to tot.  One barrier.
-  __device__ __forceinline__ void reduce_publish(T s0, T s1, T s2) const {
-    s0 = wave_sum(s0);
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
+  // workgroup they go straight to tot.  One barrier.  With `placement`, the
+  // second and third scalars are replaced by x and x^2, x = 1 + this XCD's id,
+  // so that after the gather every rank can tell (identically) whether all ranks
+  // share one XCD: G * sum(x^2) == sum(x)^2 exactly when all x are equal.
+  __device__ __forceinline__ void reduce_publish(T s0, T s1, T s2, bool placement = false) const {
+    wave_sum3(s0, s1, s2);
     const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
       wsum[w * 3 + 0] = s0;
@@ -107,27 +110,24 @@ struct RegItem {
       wsum[w * 3 + 2] = s2;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    T t0 = wsum[0], t1 = wsum[1], t2 = wsum[2];
+    // lanes 0..2 of wave 0: scalar l summed over the waves in order, published
+    const int l = threadIdx.x;
+    if (l >= 3) return;
+    T tv = wsum[l];
 #pragma unroll
-    for (int i = 1; i < NWAVES; ++i) {
-      t0 += wsum[i * 3 + 0];
-      t1 += wsum[i * 3 + 1];
-      t2 += wsum[i * 3 + 2];
-    }
+    for (int i = 1; i < NWAVES; ++i) tv += wsum[i * 3 + l];
     if (G == 1) {
-      tot[0] = t0;
-      tot[1] = t1;
-      tot[2] = t2;
+      tot[l] = tv;
       return;
+    }
+    if (placement && l > 0) {
+      const T x = (T)(1 + xcc_id());
+      tv = l == 1 ? x : x * x;
     }
     const uint32_t epoch = base + seq;
     uint64_t *r = rec((int)(seq & 1u), rank);
-    const T tv[3] = {t0, t1, t2};
 #pragma unroll
-    for (int s = 0; s < 3; ++s)
-#pragma unroll
-      for (int qq = 0; qq < WPV; ++qq) xstore(r + s * WPV + qq, payload_word(tv[s], qq), epoch);
+    for (int qq = 0; qq < WPV; ++qq) xstore_sel(r + l * WPV + qq, payload_word(tv, qq), epoch, xcd);
   }
 
   static __device__ __forceinline__ T word_value(const uint32_t *w, int v) {
@@ -146,9 +146,9 @@ struct RegItem {
   __device__ __forceinline__ bool gather(T *vec, int edge, T &t0, T &t1, T &t2) {
     if (G == 1) {
       __syncthreads();
-      t0 = tot[0];
-      t1 = tot[1];
-      t2 = tot[2];
+      t0 = uniform(tot[0]);
+      t1 = uniform(tot[1]);
+      t2 = uniform(tot[2]);
       ++seq;
       return true;
     }
@@ -208,10 +208,10 @@ struct RegItem {
     if (!ok) ctl[0] = 1;
     __syncthreads();
     ++seq;
-    t0 = tot[0];
-    t1 = tot[1];
-    t2 = tot[2];
-    return ctl[0] == 0;
+    t0 = uniform(tot[0]);
+    t1 = uniform(tot[1]);
+    t2 = uniform(tot[2]);
+    return __builtin_amdgcn_readfirstlane(ctl[0]) == 0;
   }
 };
 
@@ -289,8 +289,11 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   it.slots = d.slots;
   it.tol = d.tol;
   it.rel = (d.flags & KF_REL) != 0;
-  // lane roles
-  const int t = threadIdx.x;
+  it.xcd = false;
+  it.xcd_ok = (d.flags & KF_NOXCD) == 0;
+  // lane roles (from an opaque thread index: recomputed per item rather than
+  // hoisted out of the batched item loop)
+  const int t = opaque_tid();
   it.q = t / R::LPR;
   it.c = t % R::LPR;
   it.m = it.q / n;
@@ -379,14 +382,19 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
     }
     if (it.owner) it.rv[own_pos] = r_i;
     it.publish_edge(r_i, 2);
-    it.reduce_publish(it.owner ? b_i * b_i : (T)0, (T)0, (T)0);
-    T gg, unused1, unused2;
-    if (!it.gather(it.rv, 2, gg, unused1, unused2)) {
+    it.reduce_publish(it.owner ? b_i * b_i : (T)0, (T)0, (T)0, true);
+    T gg, sx, sxx;
+    if (!it.gather(it.rv, 2, gg, sx, sxx)) {
       out.status = GBDPCG_TIMED_OUT;
       timed_out = true;
       if (pass == 0) fo = out; else ao = out;
       break;
     }
+    // the sums are of small integers, exact in T; compare in double
+    // (uniform by construction; readfirstlane lets the compiler branch on it in
+    // scalar code)
+    it.xcd = __builtin_amdgcn_readfirstlane(
+                 (int)(it.xcd_ok && it.G > 1 && (double)sx * (double)sx == (double)it.G * (double)sxx)) != 0;
     // exit threshold: |r'z| < tol (reference) or r'r <= tol^2 rhs'rhs
     const double thr = it.rel ? it.tol * it.tol * (double)gg : it.tol;
     T r_g = it.ghost ? it.rv[it.gpos] : (T)0;
