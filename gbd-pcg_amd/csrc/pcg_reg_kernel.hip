@@ -71,28 +71,48 @@ struct RegItem {
 
   // Owner lanes publish their value into the edge slots of the current exchange
   // (value kRecScalars + e*n + row for own knot a+e, kRecScalars + 2n + e*n + row
-  // for own knot b-1-e).
-  __device__ __forceinline__ void publish_edge(T val, int edge) const {
-    if (!owner || G == 1) return;
+  // for own knot b-1-e).  A lane has at most two such slots (K >= 2), slot_lo for
+  // its left-edge role and slot_hi for its right-edge one, each -1 when absent and
+  // tagged with its depth e in bit 16 (edge = 1 publishes depth 0 only).
+  int slot_lo, slot_hi;
+
+  __device__ __forceinline__ void set_edge_slots() {
     const int kl = m - 1;
-    const uint32_t epoch = base + seq;
-    uint64_t *r = rec((int)(seq & 1u), rank);
-    int rw = row;  // recomputed per call: the slot addresses need not stay live
-    asm volatile("" : "+v"(rw));
+    slot_lo = -1;
+    slot_hi = -1;
+    if (!owner) return;
+    if (kl < 2) slot_lo = (kRecScalars + kl * n + row) | (kl << 16);
+    const int e = K - 1 - kl;
+    if (e < 2) slot_hi = (kRecScalars + 2 * n + e * n + row) | (e << 16);
+  }
+
+  template <bool XCD>
+  __device__ __forceinline__ void store_edges(uint64_t *r, T val, int edge, uint32_t epoch) const {
+    const int lim = edge << 16;
+    if (slot_lo >= 0 && slot_lo < lim) {
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      if (e >= edge) break;
-      if (kl == e) {
-#pragma unroll
-        for (int qq = 0; qq < WPV; ++qq)
-          xstore_sel(r + (kRecScalars + e * n + rw) * WPV + qq, payload_word(val, qq), epoch, xcd);
-      }
-      if (kl == K - 1 - e) {
-#pragma unroll
-        for (int qq = 0; qq < WPV; ++qq)
-          xstore_sel(r + (kRecScalars + 2 * n + e * n + rw) * WPV + qq, payload_word(val, qq), epoch, xcd);
+      for (int qq = 0; qq < WPV; ++qq) {
+        uint64_t *p = r + (slot_lo & 0xFFFF) * WPV + qq;
+        if constexpr (XCD) xstore_xcd(p, payload_word(val, qq), epoch); else xstore(p, payload_word(val, qq), epoch);
       }
     }
+    if (slot_hi >= 0 && slot_hi < lim) {
+#pragma unroll
+      for (int qq = 0; qq < WPV; ++qq) {
+        uint64_t *p = r + (slot_hi & 0xFFFF) * WPV + qq;
+        if constexpr (XCD) xstore_xcd(p, payload_word(val, qq), epoch); else xstore(p, payload_word(val, qq), epoch);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void publish_edge(T val, int edge) const {
+    if (G == 1) return;
+    const uint32_t epoch = base + seq;
+    uint64_t *r = rec((int)(seq & 1u), rank);
+    if (xcd)
+      store_edges<true>(r, val, edge, epoch);
+    else
+      store_edges<false>(r, val, edge, epoch);
   }
 
   // Workgroup totals of three per-lane partials (DPP per wave, then the waves in
@@ -197,7 +217,7 @@ struct RegItem {
           dst = reinterpret_cast<uint32_t *>(vec + (K + 2 + e) * nP + rr) + qq;
         }
         uint32_t val = 0;
-        if (xwait_lazy(src, epoch, val)) {
+        if (xcd ? xwait(src, epoch, val) : xwait_lazy(src, epoch, val)) {
           *dst = val;
         } else {
           ok = false;
@@ -303,6 +323,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   it.act_w = it.m >= 1 && it.m <= K;
   it.owner = it.act_w && it.c == 0;
   it.ghost = it.c == R::LPR - 1 && it.q < 4 * n;
+  it.set_edge_slots();
   {
     const int gk = it.q / n, gr = it.q - (it.q / n) * n;  // ghost element q: knot slot 0..3
     const int gj = gk < 2 ? gk : K + gk;                  // local knots 0, 1, K+2, K+3
