@@ -118,6 +118,7 @@ __device__ __forceinline__ bool xwait_words(uint64_t *p, uint32_t epoch, uint32_
   return false;
 }
 
+
 // a / b for the scalar recurrences.  FP32: hardware reciprocal (1 ulp) and one
 // residual correction of the quotient, about 1 ulp in all (no denormal-mode
 // switches, unlike the IEEE sequence, and four dependent operations instead of

@@ -235,27 +235,29 @@ struct RegItem {
   }
 };
 
+// Four accumulator chains (the FMA latency, not the issue rate, bounds a short
+// dot product), summed in a fixed order.
 template <typename T, int CW>
 __device__ __forceinline__ T chunk_dot(const T (&mat)[CW], const T *__restrict__ x) {
-  T a0 = 0, a1 = 0;
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
 #pragma unroll
   for (int j = 0; j < CW; j += 4) {
     if constexpr (sizeof(T) == 4) {
       const float4 xv = *reinterpret_cast<const float4 *>(x + j);
       a0 = __builtin_fmaf(mat[j + 0], xv.x, a0);
       a1 = __builtin_fmaf(mat[j + 1], xv.y, a1);
-      a0 = __builtin_fmaf(mat[j + 2], xv.z, a0);
-      a1 = __builtin_fmaf(mat[j + 3], xv.w, a1);
+      a2 = __builtin_fmaf(mat[j + 2], xv.z, a2);
+      a3 = __builtin_fmaf(mat[j + 3], xv.w, a3);
     } else {
       const double2 x0 = *reinterpret_cast<const double2 *>(x + j);
       const double2 x1 = *reinterpret_cast<const double2 *>(x + j + 2);
       a0 = fmaT(mat[j + 0], x0.x, a0);
       a1 = fmaT(mat[j + 1], x0.y, a1);
-      a0 = fmaT(mat[j + 2], x1.x, a0);
-      a1 = fmaT(mat[j + 3], x1.y, a1);
+      a2 = fmaT(mat[j + 2], x1.x, a2);
+      a3 = fmaT(mat[j + 3], x1.y, a3);
     }
   }
-  return a0 + a1;
+  return (a0 + a1) + (a2 + a3);
 }
 
 // Sum over the LPR lanes sharing one matrix row; all of them get the same bits.
@@ -291,6 +293,8 @@ __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A
 }
 
 __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
+__device__ __forceinline__ bool finite_t(float v) { return __builtin_isfinite(v); }
+__device__ __forceinline__ bool finite_t(double v) { return __builtin_isfinite(v); }
 
 
 template <typename T, int NS, int NT>
@@ -496,6 +500,9 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
           timed_out = true;
           break;
         }
+        // ghost lanes: the neighbour's w, read now so the LDS latency overlaps the
+        // scalar recurrence below
+        const T w_g = it.ghost ? it.wv[it.gpos] : (T)0;
         ++iter;
         out.eta = (double)eta_new;
         out.rr = (double)rr;
@@ -503,14 +510,14 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
           out.status = GBDPCG_CONVERGED;
           break;
         }
-        if (!finite_d((double)eta_new)) {
+        if (!finite_t(eta_new)) {
           out.status = GBDPCG_NONFINITE;
           break;
         }
         // beta and eta'/alpha are independent: two divisions in flight at once
         const T beta = qdiv(eta_new, eta);
         const T den = delta_new - beta * qdiv(eta_new, alpha);
-        if (den == (T)0 || !finite_d((double)den)) {
+        if (den == (T)0 || !finite_t(den)) {
           out.status = GBDPCG_BREAKDOWN;
           break;
         }
@@ -518,7 +525,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
         eta = eta_new;
         p_i = fmaT(beta, p_i, u_i);
         s_i = fmaT(beta, s_i, w_i);
-        if (it.ghost) s_g = fmaT(beta, s_g, it.wv[it.gpos]);
+        if (it.ghost) s_g = fmaT(beta, s_g, w_g);
       }
       out.iters = (int)iter;
     }
