@@ -215,6 +215,8 @@ struct Ctx {
   uint32_t seq;    // exchanges done so far in this item
   int edge;        // edge knots carried per side (1 reference, 2 fused)
   bool fused, rel;
+  bool xcd;        // every rank on this workgroup's XCD: exchange stores stay in L2
+  bool xcd_ok;     // the system allows XCD-local stores (KF_NOXCD clear)
 };
 
 // Publishes this rank's record for the next exchange: scalars s0..s2 and the
@@ -244,7 +246,10 @@ __device__ __forceinline__ void publish(Ctx<T> &c, T s0, T s1, T s2, const T *ve
     // record layout (value index -> words [vi*WPV, vi*WPV + WPV)): scalars at 0..2,
     // left edge at kRecScalars + e*n + i, right edge at kRecScalars + 2n + e*n + i
     const int slot_vi = vi < 3 ? vi : (vi < 3 + en ? kRecScalars + (vi - 3) : kRecScalars + 2 * g.n + (vi - 3 - en));
-    xstore(rec + slot_vi * WPV + q, payload_word(val, q), epoch);
+    if (c.xcd)
+      xstore_xcd(rec + slot_vi * WPV + q, payload_word(val, q), epoch);
+    else
+      xstore(rec + slot_vi * WPV + q, payload_word(val, q), epoch);
   }
 }
 
@@ -345,9 +350,16 @@ __device__ __forceinline__ CoreOut pcg_core(Ctx<T> &c, const T *__restrict__ rhs
     gg += b * b;
   });
   block_sum3(gg, t1, t2, c.wsum);
-  publish(c, gg, (T)0, (T)0, r);
+  // the first exchange also carries x = 1 + this XCD's id and x^2: all ranks share
+  // one XCD exactly when G * sum(x^2) == sum(x)^2 (small integers, exact in T)
+  const T xid = (T)(1 + xcc_id());
+  publish(c, gg, xid, xid * xid, r);
   if (!gather(c, r)) { out.status = GBDPCG_TIMED_OUT; return out; }
   gg = sum_ranks(c.part, g.G, 0);
+  {
+    const double sx = (double)sum_ranks(c.part, g.G, 1), sxx = (double)sum_ranks(c.part, g.G, 2);
+    c.xcd = __builtin_amdgcn_readfirstlane((int)(c.xcd_ok && g.G > 1 && sx * sx == (double)g.G * sxx)) != 0;
+  }
 
   T eta, rr, alpha = 0, beta = 0, delta = 0;
   if (!c.fused) {
@@ -559,6 +571,8 @@ __device__ __forceinline__ void run_item(const KSys &d, uint32_t rank, T *lds) {
   c.fused = (d.flags & KF_FUSED) != 0;
   c.rel = (d.flags & KF_REL) != 0;
   c.edge = c.fused ? 2 : 1;
+  c.xcd = false;
+  c.xcd_ok = (d.flags & KF_NOXCD) == 0;
   const size_t mat = (size_t)g.K * g.n * g.RS;
   c.Sm = lds;
   c.Pm = c.Sm + mat;
