@@ -169,9 +169,9 @@ def main():
         graph = torch.cuda.CUDAGraph()
         gs = torch.cuda.Stream()
         torch.cuda.synchronize()
+        lams64 = [d["lam"] for d in systems]
         with torch.cuda.graph(graph, stream=gs):
-            for d in systems:
-                d["lam"].zero_()
+            torch._foreach_zero_(lams64)  # one launch resets every lambda0
             plan.run(stream=gs)
         for _ in range(3):
             graph.replay()
