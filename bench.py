@@ -71,6 +71,20 @@ def main():
     gbdpcg.precond_device(S, P, n, N, gbdpcg.PRECOND_SS)
     kw = dict(tol=tol, max_iter=max_iter, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
 
+    # on-device symmetric-stair Pinv formation (the documented call without Pinv pays it)
+    Ptmp = torch.empty_like(S)
+    for _ in range(3):
+        gbdpcg.precond_device(S, Ptmp, n, N, gbdpcg.PRECOND_SS)
+    pa, pb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    pa.record(stream)
+    for _ in range(20):
+        gbdpcg.precond_device(S, Ptmp, n, N, gbdpcg.PRECOND_SS)
+    pb.record(stream)
+    torch.cuda.synchronize()
+    precond_us = pa.elapsed_time(pb) * 1000.0 / 20
+    del Ptmp
+
     # the autotuner picks the layout of this shape (timed on these inputs)
     tune = None if args.no_tune else gbdpcg.tune_single(S, g, n, N, P, **kw)
 
@@ -152,16 +166,19 @@ def main():
     batched = None
     if not args.no_batched:
         B = 64
-        systems = []
+        systems, host64 = [], []
         for b in range(B):
             Sb_h, gb_h = gbdpcg.lqr_generate(n, m, N, 4000 + b, dtype=np.float32)
             Sb = torch.from_numpy(Sb_h).to(dev)
             Pb = torch.empty_like(Sb)
             gbdpcg.precond_device(Sb, Pb, n, N, gbdpcg.PRECOND_SS)
+            glam_h = gbdpcg.randn(n * N, 7000 + b)
             systems.append(dict(S=Sb, Pinv=Pb, gamma=torch.from_numpy(gb_h).to(dev),
                                 lam=torch.zeros(n * N, device=dev), n=n, N=N, tol=tol, max_iter=max_iter,
-                                glam=torch.from_numpy(gbdpcg.randn(n * N, 7000 + b)).to(dev),
+                                glam=torch.from_numpy(glam_h).to(dev),
                                 ggamma=torch.zeros(n * N, device=dev), gS=torch.zeros_like(Sb), adjoint=True))
+            host64.append((Sb_h.astype(np.float64), Pb.cpu().numpy().astype(np.float64), gb_h.astype(np.float64),
+                           glam_h.astype(np.float64)))
         plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
                             autotune=not args.no_tune)
         plan.run()
@@ -189,7 +206,8 @@ def main():
                    "mean_iters_fwd": float(np.mean([r.iters for r in bres])),
                    "mean_iters_adj": float(np.mean([r.adj_iters for r in bres])),
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
-                   "plan": plan.info(), "tune": plan.tune_report}
+                   "plan": plan.info(), "tune": plan.tune_report,
+                   "cpu_baseline": cpu_batched(np, host64, n, N, tol, max_iter, args.cpu_seconds)}
 
     workloads = None
     if not args.no_workloads:
@@ -234,9 +252,36 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
         "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
-        "reference_recurrence_us": round(ref_us, 2), "tune": tune, "batched": batched, "workloads": workloads,
+        "reference_recurrence_us": round(ref_us, 2), "precond_ss_us": round(precond_us, 2), "tune": tune, "batched": batched, "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
+
+
+def cpu_batched(np, host, n, N, tol, max_iter, seconds):
+    """The same 64 forward + adjoint solves on the host: the FP64 restatement of
+    the reference iteration (oracle/), one system per thread on the host's cores
+    (the C solver releases the interpreter lock), for a bounded time."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    zero = np.zeros(n * N)
+
+    def one(i):
+        S, P, g, gl = host[i]
+        O.pcg(S, P, g, zero, n, N, tol, max_iter, O.EXIT_REL)
+        O.pcg(S, P, gl, zero, n, N, tol, max_iter, O.EXIT_REL)
+        return 2
+
+    done, t0 = 0, time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        while True:
+            done += sum(ex.map(one, range(len(host))))
+            if time.perf_counter() - t0 >= seconds:
+                break
+    el = time.perf_counter() - t0
+    return {"value": round(done / el, 1), "unit": "fwd+adj solves/s", "cores": threads, "kind": "port",
+            "sample": f"{done // 2} forward + adjoint FP64 solve pairs of the 64 n=14,N=128 systems "
+                      f"(oracle/pcg_oracle.c), {threads} threads, {el:.1f} s"}
 
 
 def _generate(gbdpcg, np, specs, seed0, dtype):

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.h"
 #include "internal.h"
 
 namespace gbdpcg {
@@ -185,8 +186,191 @@ __global__ void __launch_bounds__(kAuxThreads) precond_kernel(const T *__restric
   }
 }
 
+// ---------------------------------------------------------------------------
+// preconditioner, n <= 32: two launches
+//   inv_diag_kernel   one wave per knot inverts theta_k in registers and writes
+//                     it to the diagonal block of Pinv row k;
+//   stair_kernel      one workgroup per knot forms the symmetric-stair
+//                     off-diagonal blocks from those inverses (or zeros them for
+//                     block-Jacobi).
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T lane_max(T v) {  // over the wave; v >= 0 or -1 (excluded)
+  v = __builtin_fmax(v, dpp<kDppRowShr1>(v));
+  v = __builtin_fmax(v, dpp<kDppRowShr2>(v));
+  v = __builtin_fmax(v, dpp<kDppRowShr4>(v));
+  v = __builtin_fmax(v, dpp<kDppRowShr8>(v));
+  v = __builtin_fmax(v, dpp<kDppRowBcast15, 0xa>(v));
+  v = __builtin_fmax(v, dpp<kDppRowBcast31, 0xc>(v));
+  return read_lane63(v);
+}
+
+template <typename T>
+__device__ __forceinline__ T lane_read(T v, int lane) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  }
+}
+
+// Gauss-Jordan with partial pivoting on [theta | I], lane r holding row r.  Rows
+// are never swapped: the lane chosen as pivot for column c is marked used, so at
+// the end its left half is the unit vector e_c and its right half is row c of the
+// inverse.  Ties in the pivot search go to the lowest row, as in a sequential
+// search for the first maximum.
+template <typename T, int NM>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) inv_diag_kernel(const T *__restrict__ S, T *__restrict__ P, int n, int N,
+                                                     int *err) {
+  const int k = blockIdx.x;
+  const int lane = threadIdx.x;
+  const size_t bs = (size_t)3 * n * n;
+  const bool is_row = lane < n;
+  T al[NM], ar[NM];  // [theta | I] row `lane`: left and right halves (two arrays stay in registers)
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    al[j] = (is_row && j < n) ? S[(size_t)k * bs + (size_t)n * n + (size_t)j * n + lane] : (T)0;
+    ar[j] = (is_row && j == lane) ? (T)1 : (T)0;
+  }
+  bool used = !is_row;
+  int mycol = 0;
+  bool ok = true;
+#pragma unroll
+  for (int col = 0; col < NM; ++col) {
+    if (col < n && ok) {
+      const T cand = used ? (T)-1 : __builtin_fabs(al[col]);
+      const T m = lane_max(cand);
+      if (!(m > (T)0)) {
+        ok = false;  // exactly zero (or NaN) pivot: singular block
+      } else {
+        const unsigned long long hit = __ballot(!used && cand == m);
+        const int pr = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
+        const T ip = (T)1 / lane_read(al[col], pr);
+        if (lane == pr) {
+#pragma unroll
+          for (int j = 0; j < NM; ++j) {
+            al[j] *= ip;
+            ar[j] *= ip;
+          }
+          used = true;
+          mycol = col;
+        }
+        const T f = lane == pr ? (T)0 : al[col];
+#pragma unroll
+        for (int j = 0; j < NM; ++j) {
+          if (j < n) {
+            al[j] = fmaT(-f, lane_read(al[j], pr), al[j]);
+            ar[j] = fmaT(-f, lane_read(ar[j], pr), ar[j]);
+          }
+        }
+      }
+    }
+  }
+  T *row = P + (size_t)k * bs;
+  if (!ok) {
+    if (lane == 0) atomicOr(err, 1);
+    for (int i = lane; i < (int)bs; i += 64) row[i] = (T)0;
+    return;
+  }
+  // diagonal block, column-major: element (mycol, j) of the inverse at j*n + mycol
+  if (is_row) {
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+      if (j < n) row[(size_t)n * n + (size_t)j * n + mycol] = ar[j];
+  }
+}
+
+// LDS: inv_k | inv_nb | phi | t  (n x n row-major each).
+template <typename T>
+__global__ void __launch_bounds__(kAuxThreads) stair_kernel(const T *__restrict__ S, T *__restrict__ P, int n, int N,
+                                                           int kind) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  T *inv_k = reinterpret_cast<T *>(lds_raw);
+  T *inv_nb = inv_k + n * n;
+  T *phi = inv_nb + n * n;
+  T *t = phi + n * n;
+  const int k = blockIdx.x;
+  const size_t bs = (size_t)3 * n * n, nn = (size_t)n * n;
+  T *row = P + (size_t)k * bs;
+  const bool ss = kind == GBDPCG_PRECOND_SS;
+  if (!ss) {
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {
+      row[i] = (T)0;
+      row[2 * nn + i] = (T)0;
+    }
+    return;
+  }
+  // inverses are stored column-major in the diagonal blocks
+  for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {
+    const int c = i / n, r = i - c * n;
+    inv_k[r * n + c] = P[(size_t)k * bs + nn + i];
+  }
+  for (int side = 0; side < 2; ++side) {
+    // side 0: (k,k-1) = -inv_k phi_k inv_{k-1};  side 1: (k,k+1) = (-inv_{k+1} phi_{k+1} inv_k)^T
+    const int nb = side == 0 ? k - 1 : k + 1;
+    T *dst = row + (side == 0 ? 0 : 2 * nn);
+    if (nb < 0 || nb >= N) {
+      for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) dst[i] = (T)0;
+      continue;
+    }
+    const int kp = side == 0 ? k : k + 1;  // phi_kp = S(kp, kp-1)
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {
+      const int c = i / n, r = i - c * n;
+      inv_nb[r * n + c] = P[(size_t)nb * bs + nn + i];
+      phi[r * n + c] = S[(size_t)kp * bs + i];
+    }
+    __syncthreads();
+    const T *left = side == 0 ? inv_k : inv_nb;   // inv of kp
+    const T *right = side == 0 ? inv_nb : inv_k;  // inv of kp-1
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {  // t = phi * right
+      const int r = i / n, c = i - r * n;
+      T acc = 0;
+      for (int q = 0; q < n; ++q) acc += phi[r * n + q] * right[q * n + c];
+      t[i] = acc;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (int)nn; i += blockDim.x) {  // -(left * t), column-major (or transposed)
+      const int r = i / n, c = i - r * n;
+      T acc = 0;
+      for (int q = 0; q < n; ++q) acc += left[r * n + q] * t[q * n + c];
+      if (side == 0)
+        dst[(size_t)c * n + r] = -acc;
+      else
+        dst[(size_t)r * n + c] = -acc;  // transpose: element (c, r) of block (k,k+1), column-major
+    }
+  }
+}
+
+template <typename T>
+static hipError_t precond_small(uint32_t n, uint32_t N, const T *S, T *P, int kind, int *err, hipStream_t stream) {
+  if (n <= 4)
+    hipLaunchKernelGGL((inv_diag_kernel<T, 4>), dim3(N), dim3(64), 0, stream, S, P, (int)n, (int)N, err);
+  else if (n <= 8)
+    hipLaunchKernelGGL((inv_diag_kernel<T, 8>), dim3(N), dim3(64), 0, stream, S, P, (int)n, (int)N, err);
+  else if (n <= 16)
+    hipLaunchKernelGGL((inv_diag_kernel<T, 16>), dim3(N), dim3(64), 0, stream, S, P, (int)n, (int)N, err);
+  else
+    hipLaunchKernelGGL((inv_diag_kernel<T, 32>), dim3(N), dim3(64), 0, stream, S, P, (int)n, (int)N, err);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const size_t lds = 4 * (size_t)n * n * sizeof(T);
+  hipLaunchKernelGGL(stair_kernel<T>, dim3(N), dim3(kAuxThreads), lds, stream, S, P, (int)n, (int)N, kind);
+  return hipGetLastError();
+}
+
 hipError_t precond_launch(int dtype, uint32_t n, uint32_t N, const void *S, void *Pinv, int kind, int *err,
                           hipStream_t stream) {
+  if (n <= 32) {
+    if (dtype == GBDPCG_F64)
+      return precond_small<double>(n, N, static_cast<const double *>(S), static_cast<double *>(Pinv), kind, err,
+                                   stream);
+    return precond_small<float>(n, N, static_cast<const float *>(S), static_cast<float *>(Pinv), kind, err, stream);
+  }
   const size_t ts = dtype == GBDPCG_F64 ? 8 : 4;
   const size_t lds = (7 * (size_t)n * n + n) * ts;
   if (dtype == GBDPCG_F64) {
