@@ -208,8 +208,9 @@ static int grow(void **p, size_t *cap, size_t need, hipStream_t s) {
 struct Layout {
   uint32_t G = 0, stride = 1, grid = 0;
   size_t lds = 0;
-  bool reg = false;  // register-resident kernel
-  int nt = 256;      // threads per workgroup
+  bool reg = false;    // register-resident kernel
+  bool fused = true;   // its recurrence (the LDS kernel takes it from the flags)
+  int nt = 256;        // threads per workgroup
 };
 
 static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
@@ -257,19 +258,20 @@ static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t w
   return GBDPCG_OK;
 }
 
-// Register-resident kernel (fused recurrence): reg_lpr(n) lanes per matrix row of
+// Register-resident kernel (either recurrence): reg_lpr(n) lanes per matrix row of
 // the owned knots plus one ghost knot each side.  Prefers 256 threads (one wave
 // per SIMD: every wave runs the uniform scalar work, so fewer waves cost less)
 // with up to 8 knots per workgroup; 512 threads when the knots need it.
 // Returns GBDPCG_ERR_UNSUPPORTED when the shape does not suit this kernel.
 static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, uint32_t kcap, Layout *L,
-                           int only_nt = 0) {
+                           int only_nt = 0, bool fused = true) {
   if (!reg_supported((int)n) || N < 4) return GBDPCG_ERR_UNSUPPORTED;
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
   for (int nt : {256, 512}) {
     if (only_nt && nt != only_nt) continue;
+    if (!fused && nt != 256) continue;  // reference-recurrence instances: 256 threads
     // FP64 rows of n = 28 need ~176 VGPRs of matrix chunks: one wave per SIMD only
     if (dtype == GBDPCG_F64 && n > 14 && nt > 256) continue;
     const uint32_t kmax = reg_max_knots(n, nt);
@@ -282,14 +284,15 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
     if (G < 1 || N / G < 2 || ceil_div(N, G) > kmax || G > (uint32_t)kMaxWgsPerSystem) continue;
     const size_t lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
     if (lds > di.max_lds) continue;
-    HIPCHK(reg_prepare(dtype, (int)n, nt, lds));
+    HIPCHK(reg_prepare(dtype, (int)n, nt, lds, fused));
     int occ = 0;
-    HIPCHK(reg_occupancy(dtype, (int)n, nt, lds, &occ));
+    HIPCHK(reg_occupancy(dtype, (int)n, nt, lds, &occ, fused));
     const uint32_t per_xcd = (uint32_t)(di.cus / kXcds) * (uint32_t)occ;
     if (occ <= 0 || G > (uint32_t)di.cus * (uint32_t)occ) continue;
     L->G = G;
     L->lds = lds;
     L->reg = true;
+    L->fused = fused;
     L->nt = nt;
     L->stride = G <= per_xcd ? (uint32_t)kXcds : 1u;
     L->grid = G * L->stride;
@@ -309,7 +312,7 @@ static std::string single_key(int dev, int dtype, uint32_t n, uint32_t N, bool f
 // Rebuilds a layout from its (kernel, threads, workgroups, stride); checks that it
 // still fits the device.
 static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint32_t G, uint32_t stride,
-                       Layout *L) {
+                       Layout *L, bool fused) {
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
@@ -319,15 +322,16 @@ static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint
   t.stride = stride;
   t.grid = G * stride;
   t.reg = reg;
+  t.fused = fused;
   t.nt = reg ? nt : kThreads;
   int occ = 0;
   if (reg) {
-    if (!reg_supported((int)n) || ceil_div(N, G) > reg_max_knots(n, nt) || N / G < 2)
+    if (!reg_supported((int)n) || ceil_div(N, G) > reg_max_knots(n, nt) || N / G < 2 || (!fused && nt != 256))
       return GBDPCG_ERR_UNSUPPORTED;
     t.lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
     if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
-    HIPCHK(reg_prepare(dtype, (int)n, nt, t.lds));
-    HIPCHK(reg_occupancy(dtype, (int)n, nt, t.lds, &occ));
+    HIPCHK(reg_prepare(dtype, (int)n, nt, t.lds, fused));
+    HIPCHK(reg_occupancy(dtype, (int)n, nt, t.lds, &occ, fused));
   } else {
     t.lds = item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype));
     if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
@@ -346,13 +350,14 @@ static int choose_layout(int dev, int dtype, uint32_t n, uint32_t N, bool fused,
   if (cfg->grid[0] == 0 && cfg->knots_per_wg == 0 && cfg->kernel == GBDPCG_KERNEL_AUTO) {
     TuneChoice c;
     if (tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
-        layout_from(dtype, n, N, c.a != 0, (int)c.b, c.c, c.d, L) == GBDPCG_OK)
+        layout_from(dtype, n, N, c.a != 0, (int)c.b, c.c, c.d, L, fused) == GBDPCG_OK)
       return GBDPCG_OK;
   }
   int rc = GBDPCG_ERR_UNSUPPORTED;
-  if (fused && cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L);
+  if (cfg->kernel != GBDPCG_KERNEL_LDS)
+    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, 0, fused);
   if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_REGISTER) {
-    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28}, N >= 4 and a fitting layout");
+    set_error("register kernel needs n in {2,6,12,14,28}, N >= 4 and a fitting layout");
     return GBDPCG_ERR_UNSUPPORTED;
   }
   if (rc == GBDPCG_ERR_UNSUPPORTED) rc = plan_single(dtype, n, N, fused, cfg->grid[0], cfg->knots_per_wg, L);
@@ -468,7 +473,7 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
   one.flags = flags | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
               exchange_flags();
   if (L.reg) {
-    HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
+    HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream, fused));
   } else {
     HIPCHK(pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
   }
@@ -669,7 +674,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   {
     Layout def;
     rc = GBDPCG_ERR_UNSUPPORTED;
-    if (fused && cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, 0, 0, &def);
+    if (cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused);
     if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel != GBDPCG_KERNEL_REGISTER)
       rc = plan_single(dtype, n, N, fused, 0, 0, &def);
     if (rc < 0) return rc;
@@ -677,7 +682,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   }
   const uint32_t Ks[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64};
   for (int reg = 1; reg >= 0; --reg) {
-    if (reg && !(fused && cfg->kernel != GBDPCG_KERNEL_LDS)) continue;
+    if (reg && cfg->kernel == GBDPCG_KERNEL_LDS) continue;
     if (!reg && cfg->kernel == GBDPCG_KERNEL_REGISTER) continue;
     for (int nt : {256, 512}) {
       if (!reg && nt != kThreads) continue;
@@ -686,7 +691,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
         if (G > (uint32_t)kMaxWgsPerSystem || (fused && N / G < 2)) continue;
         for (uint32_t stride : {(uint32_t)kXcds, 1u}) {
           Layout L;
-          if (layout_from(dtype, n, N, reg != 0, nt, G, stride, &L) == GBDPCG_OK) add(L);
+          if (layout_from(dtype, n, N, reg != 0, nt, G, stride, &L, fused) == GBDPCG_OK) add(L);
         }
       }
     }
@@ -756,7 +761,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     for (uint32_t t = 0; t <= trials; ++t) {
       if ((e = hipMemsetAsync(lam, 0, (size_t)n * N * ts, st)) != hipSuccess) return fail("hipMemsetAsync", e);
       if ((e = hipEventRecord(ea, st)) != hipSuccess) return fail("hipEventRecord", e);
-      e = L.reg ? reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st)
+      e = L.reg ? reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st, fused)
                 : pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st);
       if (e != hipSuccess) return fail("solve launch", e);
       if ((e = hipEventRecord(eb, st)) != hipSuccess) return fail("hipEventRecord", e);

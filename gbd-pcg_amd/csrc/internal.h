@@ -139,10 +139,14 @@ hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t coun
 bool reg_supported(int n);
 size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt);
 uint32_t reg_max_knots(uint32_t n, int nt);
-hipError_t reg_prepare(int dtype, int n, int nt, size_t lds);
-hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu);
+// fused = false selects the reference-recurrence instances (256 threads only)
+hipError_t reg_prepare(int dtype, int n, int nt, size_t lds, bool fused = true);
+hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu, bool fused = true);
 hipError_t reg_launch(int dtype, int n, int nt, const KSys &one, const KSys *systems, const uint32_t *items,
-                      uint32_t rounds, uint32_t stride, uint32_t grid, size_t lds, hipStream_t stream);
+                      uint32_t rounds, uint32_t stride, uint32_t grid, size_t lds, hipStream_t stream,
+                      bool fused = true);
+// pcg_reg_ref_kernel.hip: reference-recurrence kernel for (dtype, n, threads), or null
+const void *reg_ref_kernel_ptr(int dtype, int n, int nt);
 // tune.cpp: autotuner choice cache (a..d: meaning fixed by the key's kind)
 struct TuneChoice {
   uint32_t a, b, c, d;

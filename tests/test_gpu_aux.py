@@ -191,8 +191,9 @@ def test_async_result_record(gpu):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [2, 6, 12, 14, 28])
 @pytest.mark.parametrize("kernel", [1, 2])
-def test_forced_kernels(gpu, dtype, n, kernel):
-    """Fused recurrence on the LDS-resident and on the register-resident kernel,
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_forced_kernels(gpu, dtype, n, kernel, recurrence):
+    """Both recurrences on the LDS-resident and on the register-resident kernel,
     each against the oracle (iterations within 1, lambda to the precision bar)."""
     import gbdpcg
     N = 64
@@ -203,14 +204,15 @@ def test_forced_kernels(gpu, dtype, n, kernel):
     lam = dev(np.zeros(n * N), dtype)
     try:
         r = gbdpcg.solve(dev(Sd, dtype), dev(gd, dtype), lam, n, N, dev(Pd, dtype), tol=tol, max_iter=400,
-                         exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, kernel=kernel)
+                         exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence, kernel=kernel)
     except gbdpcg.GbdpcgError as e:
         # the register kernel holds FP64 rows of n > 14 only at 256 threads; it is
         # allowed to decline, but only with UNSUPPORTED
         assert kernel == 2 and e.code == gbdpcg.ERR_UNSUPPORTED
         return
     assert r.status == gbdpcg.CONVERGED
-    _, it_same, *_ = O.pcg(Sd, Pd, gd, np.zeros(n * N, dtype), n, N, tol, 400, O.EXIT_REL, dtype, fused=True)
+    _, it_same, *_ = O.pcg(Sd, Pd, gd, np.zeros(n * N, dtype), n, N, tol, 400, O.EXIT_REL, dtype,
+                           fused=recurrence == 1)
     assert abs(r.iters - it_same) <= 1
     lam_ref, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 0.0, r.iters)
     assert rel(lam.cpu().numpy(), lam_ref) <= (1e-4 if dtype == np.float32 else 1e-10)
