@@ -155,7 +155,7 @@ def main():
     counters = os.path.join(ROOT, "profiles", "pcg_reg_kernel_n14_N128_counters.json")
     if os.path.exists(counters):
         with open(counters) as f:
-            rec = json.load(f)["kernels"].get("gbdpcg::pcg_reg_kernel<float, 14, 256>")
+            rec = json.load(f)["kernels"].get("gbdpcg::pcg_reg_kernel<float, 14, 256, true>")
         if rec:
             traffic = int(rec["traffic_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
