@@ -98,21 +98,29 @@ def main():
     for i in range(W):
         gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw)
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * K)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t_start.record(stream)
     for i in range(K):
-        ev[2 * i].record(stream)
         gbdpcg.solve_async(S, g, lams[W + i], n, N, P, dres, **kw)
-        ev[2 * i + 1].record(stream)
     t_end.record(stream)
     torch.cuda.synchronize()
     total_ms = t_start.elapsed_time(t_end)
     us_per_solve = total_ms * 1000.0 / K
-    kern_us = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(K)])) * 1000.0
     lam_last = lams[W + K - 1].cpu().numpy()
+    # the solve kernel's own duration: HIP events around each launch on the launch
+    # stream (a second pass; the events would otherwise count in the steps above)
+    kn = min(K, 200)
+    lams[:kn].zero_()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * kn)]
+    torch.cuda.synchronize()
+    for i in range(kn):
+        ev[2 * i].record(stream)
+        gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw)
+        ev[2 * i + 1].record(stream)
+    torch.cuda.synchronize()
+    kern_us = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(kn)])) * 1000.0
 
     # end-to-end drop-in call (launch + wait + iteration count back to the host)
     lam_tmp = torch.zeros(n * N, device=dev)

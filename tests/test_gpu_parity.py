@@ -159,3 +159,40 @@ def test_warm_start_and_repeat(gpu):
     # warm start from the solution: converges in at most a couple of iterations
     lam2, res2 = gpu_solve(S, P, g, first[0], n, N, dtype, tol=1e-5, max_iter=173, exit_mode=1)
     assert res2.iters <= 2
+
+
+def test_async_solve_graph_capture(gpu):
+    """Stream-ordered solves captured in a graph (after one eager call sizes the
+    workspace) and replayed with eager work in between: every replay equals the
+    eager sequence bit for bit."""
+    import torch
+    import gbdpcg
+    n, N, dtype = 14, 96, np.float32
+    S, P, g = lqr_system(n, N, 404, dtype)
+    dS, dP, dg = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (S, P, g))
+    kw = dict(tol=1e-5, max_iter=173, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+    stream = torch.cuda.Stream()
+    lams = [torch.zeros(n * N, device="cuda") for _ in range(3)]
+    recs = [torch.zeros(gbdpcg.RESULT_BYTES, dtype=torch.uint8, device="cuda") for _ in range(3)]
+    gs = [dg, dg * 0.5, -dg]
+    with torch.cuda.stream(stream):
+        for lam, rec, gg in zip(lams, recs, gs):  # eager: sizes the stream's workspace
+            gbdpcg.solve_async(dS, gg, lam, n, N, dP, rec, stream=stream, **kw)
+    torch.cuda.synchronize()
+    eager = [l.cpu().numpy().copy() for l in lams]
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        for lam in lams:
+            lam.zero_()
+        for lam, rec, gg in zip(lams, recs, gs):
+            gbdpcg.solve_async(dS, gg, lam, n, N, dP, rec, stream=stream, **kw)
+    for _ in range(3):
+        junk = torch.randn(1 << 16, device="cuda")
+        junk.mul_(2.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        for lam, e, rec in zip(lams, eager, recs):
+            assert np.array_equal(lam.cpu().numpy(), e)
+            r = gbdpcg.Result.from_buffer_copy(rec.cpu().numpy().tobytes())
+            assert r.status == gbdpcg.CONVERGED
