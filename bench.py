@@ -152,6 +152,28 @@ def main():
     ref_us = t_start.elapsed_time(t_end) * 1000.0 / nref
     lams[:W + nref].zero_()
 
+    # the other single-solve sizes of the same problem family (n=14, m=7), tuned,
+    # back to back from lambda0 = 0
+    sweep = {}
+    for Ns in (32, 64, 128, 256):
+        Ss_h, gs_h = gbdpcg.lqr_generate(n, m, Ns, 2000 + Ns, dtype=np.float32)
+        Ss, gs = torch.from_numpy(Ss_h).to(dev), torch.from_numpy(gs_h).to(dev)
+        Ps = torch.empty_like(Ss)
+        gbdpcg.precond_device(Ss, Ps, n, Ns, gbdpcg.PRECOND_SS)
+        if not args.no_tune:
+            gbdpcg.tune_single(Ss, gs, n, Ns, Ps, **kw)
+        ls = torch.zeros((60, n * Ns), device=dev)
+        for i in range(10):
+            gbdpcg.solve_async(Ss, gs, ls[i], n, Ns, Ps, dres, **kw)
+        torch.cuda.synchronize()
+        t_start.record(stream)
+        for i in range(10, 60):
+            gbdpcg.solve_async(Ss, gs, ls[i], n, Ns, Ps, dres, **kw)
+        t_end.record(stream)
+        torch.cuda.synchronize()
+        rs = gbdpcg.solve(Ss, gs, torch.zeros(n * Ns, device=dev), n, Ns, Ps, **kw)
+        sweep[f"N{Ns}"] = {"us_per_solve": round(t_start.elapsed_time(t_end) * 1000.0 / 50, 2), "iters": rs.iters}
+
     # roofline of the solve kernel: algorithmic HBM bytes per launch (S and Pinv read
     # once, gamma and lambda0 read, lambda written)
     ts = 4
@@ -260,7 +282,8 @@ def main():
         "roofline": roofline, "cpu_baseline": cpu,
         "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
-        "reference_recurrence_us": round(ref_us, 2), "precond_ss_us": round(precond_us, 2), "tune": tune, "batched": batched, "workloads": workloads,
+        "reference_recurrence_us": round(ref_us, 2), "precond_ss_us": round(precond_us, 2),
+        "n14_sizes": sweep, "tune": tune, "batched": batched, "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
 
