@@ -274,7 +274,7 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
     if (!fused && nt != 256) continue;  // reference-recurrence instances: 256 threads
     // FP64 rows of n = 28 need ~176 VGPRs of matrix chunks: one wave per SIMD only
     if (dtype == GBDPCG_F64 && n > 14 && nt > 256) continue;
-    const uint32_t kmax = reg_max_knots(n, nt);
+    const uint32_t kmax = reg_max_knots(dtype, n, nt);
     if (kmax < 2) continue;
     uint32_t G = want_G;
     if (!G) {  // about 8 knots (or the cap) per workgroup, never fewer than 2
@@ -326,7 +326,8 @@ static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint
   t.nt = reg ? nt : kThreads;
   int occ = 0;
   if (reg) {
-    if (!reg_supported((int)n) || ceil_div(N, G) > reg_max_knots(n, nt) || N / G < 2 || (!fused && nt != 256))
+    if (!reg_supported((int)n) || ceil_div(N, G) > reg_max_knots(dtype, n, nt) || N / G < 2 ||
+        (!fused && nt != 256))
       return GBDPCG_ERR_UNSUPPORTED;
     t.lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
     if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;

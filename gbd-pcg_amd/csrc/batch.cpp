@@ -148,7 +148,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   const uint32_t kcap = cfg->knots_per_wg ? std::max<uint32_t>(cfg->knots_per_wg, 1u) : 0u;
   for (uint32_t i = 0; use_reg && i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
-    const uint32_t kmax = reg_max_knots(d.n, nt);
+    const uint32_t kmax = reg_max_knots(dtype, d.n, nt);
     const uint32_t target = kcap ? std::min(kmax, std::max<uint32_t>(kcap, 2u)) : kmax;
     uint32_t g = d.wgs;
     if (!g) {  // the cap is a target: every workgroup keeps at least 2 knots

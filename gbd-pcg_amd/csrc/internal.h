@@ -57,9 +57,14 @@ __host__ __device__ constexpr uint32_t col_chunk(uint32_t n) {
 __host__ __device__ constexpr uint32_t row_stride(uint32_t n) { return col_chunk(n) * kColSplit; }
 
 // Register kernel: lanes sharing one matrix row, and the columns each holds.
-__host__ __device__ constexpr uint32_t reg_lpr(uint32_t n) { return n <= 2 ? 1u : 2u; }
-__host__ __device__ constexpr uint32_t reg_chunk(uint32_t n) {
-  return 4u * ((3u * padded_n(n) + 4u * reg_lpr(n) - 1u) / (4u * reg_lpr(n)));
+// Two lanes per row halve each lane's matrix registers; one lane per row packs
+// twice the knots into a workgroup, which is what the 512-thread FP32 n = 28
+// instance needs (its batches are bound by knots resident per CU).
+__host__ __device__ constexpr uint32_t reg_lpr(uint32_t n, int nt, int tsize) {
+  return n <= 2 ? 1u : (n == 28 && nt == 512 && tsize == 4 ? 1u : 2u);
+}
+__host__ __device__ constexpr uint32_t reg_chunk(uint32_t n, uint32_t lpr) {
+  return 4u * ((3u * padded_n(n) + 4u * lpr - 1u) / (4u * lpr));
 }
 
 // Words (64-bit) of one exchange record: kRecScalars scalar slots (3 per wave of
@@ -138,7 +143,7 @@ hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t coun
 // pcg_reg_kernel.hip (register-resident latency kernel, fused recurrence)
 bool reg_supported(int n);
 size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt);
-uint32_t reg_max_knots(uint32_t n, int nt);
+uint32_t reg_max_knots(int dtype, uint32_t n, int nt);
 // fused = false selects the reference-recurrence instances (256 threads only)
 hipError_t reg_prepare(int dtype, int n, int nt, size_t lds, bool fused = true);
 hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu, bool fused = true);

@@ -50,8 +50,8 @@ template <typename T, int NS, int NT>
 struct RegItem {
   static constexpr int n = NS;
   static constexpr int nP = (int)padded_n(NS);
-  static constexpr int LPR = (int)reg_lpr(NS);  // lanes per matrix row
-  static constexpr int CW = (int)reg_chunk(NS);  // columns per lane
+  static constexpr int LPR = (int)reg_lpr(NS, NT, (int)sizeof(T));  // lanes per matrix row
+  static constexpr int CW = (int)reg_chunk(NS, LPR);                 // columns per lane
   static constexpr int WPV = (int)(sizeof(T) / 4);
   static constexpr int RW = (int)rec_words(NS, WPV);
   static constexpr int NWAVES = NT / 64;

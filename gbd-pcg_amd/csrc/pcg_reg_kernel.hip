@@ -42,14 +42,15 @@ bool reg_supported(int n) { return n == 2 || n == 6 || n == 12 || n == 14 || n =
 
 size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt) {
   const size_t ts = dtype == GBDPCG_F64 ? 8 : 4;
-  const size_t VL = (size_t)(K + 4) * padded_n(n) + reg_lpr(n) * reg_chunk(n);
+  const uint32_t lpr = reg_lpr(n, nt, (int)ts);
+  const size_t VL = (size_t)(K + 4) * padded_n(n) + lpr * reg_chunk(n, lpr);
   return (5 * VL + 3 * (size_t)(nt / 64) + 4 + 4) * ts;
 }
 
 // Largest knots per workgroup the lane layout allows: reg_lpr(n) lanes per row
 // of the owned knots plus one ghost knot each side, and 4n ghost lanes.
-uint32_t reg_max_knots(uint32_t n, int nt) {
-  const uint32_t rows = (uint32_t)nt / reg_lpr(n);
+uint32_t reg_max_knots(int dtype, uint32_t n, int nt) {
+  const uint32_t rows = (uint32_t)nt / reg_lpr(n, nt, dtype == GBDPCG_F64 ? 8 : 4);
   if (rows < 4 * n) return 0;
   return rows / n - 2;
 }
