@@ -127,17 +127,18 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   // stateSize in the specialised set, at least 4 knots), else LDS-resident rows
   bool use_reg = fused_cfg && cfg->kernel != GBDPCG_KERNEL_LDS;
   bool same_n = true;
+  bool f64_wide = false;  // FP64 rows of n = 28: only the 256-thread instance holds them
   for (uint32_t i = 0; i < count; ++i) {
     if (!reg_supported((int)systems[i].n) || systems[i].N < 4) use_reg = false;
-    // FP64 rows of n = 28 do not fit the registers of a 512-thread workgroup
-    if (dtype == GBDPCG_F64 && systems[i].n > 14) use_reg = false;
+    if (dtype == GBDPCG_F64 && systems[i].n > 14) f64_wide = true;
     if (systems[i].n != systems[0].n) same_n = false;
   }
+  if (f64_wide && !same_n) use_reg = false;
   if (cfg->kernel == GBDPCG_KERNEL_REGISTER && !use_reg) {
     set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28} and N >= 4 for every system");
     return GBDPCG_ERR_UNSUPPORTED;
   }
-  const int nt = (use_reg && same_n && cfg->block[0] == 256) ? 256 : 512;
+  const int nt = (use_reg && same_n && (cfg->block[0] == 256 || f64_wide)) ? 256 : 512;
   const int kn = same_n ? (int)systems[0].n : 0;
 
   // workgroups per system and the launch's LDS size

@@ -58,10 +58,11 @@ __host__ __device__ constexpr uint32_t row_stride(uint32_t n) { return col_chunk
 
 // Register kernel: lanes sharing one matrix row, and the columns each holds.
 // Two lanes per row halve each lane's matrix registers; one lane per row packs
-// twice the knots into a workgroup, which is what the 512-thread FP32 n = 28
-// instance needs (its batches are bound by knots resident per CU).
+// twice the knots into a workgroup, which is what n = 28 needs (its batches are
+// bound by knots resident per CU): FP32 at 512 threads (~250 VGPRs), FP64 at 256
+// threads (one wave per SIMD, matrix chunks partly in AGPRs).
 __host__ __device__ constexpr uint32_t reg_lpr(uint32_t n, int nt, int tsize) {
-  return n <= 2 ? 1u : (n == 28 && nt == 512 && tsize == 4 ? 1u : 2u);
+  return n <= 2 ? 1u : (n == 28 && ((nt == 512 && tsize == 4) || (nt == 256 && tsize == 8)) ? 1u : 2u);
 }
 __host__ __device__ constexpr uint32_t reg_chunk(uint32_t n, uint32_t lpr) {
   return 4u * ((3u * padded_n(n) + 4u * lpr - 1u) / (4u * lpr));
