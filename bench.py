@@ -52,7 +52,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the solver runs each system on one "
-              "device; see DESIGN.md for how the batched path would shard across GPUs.", file=sys.stderr)
+              "device (a batch would shard by systems, one plan per GPU, with no exchange between them).",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
