@@ -261,7 +261,8 @@ static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t w
 // Register-resident kernel (either recurrence): reg_lpr(n) lanes per matrix row of
 // the owned knots plus one ghost knot each side.  Prefers 256 threads (one wave
 // per SIMD: every wave runs the uniform scalar work, so fewer waves cost less)
-// with up to 8 knots per workgroup; 512 threads when the knots need it.
+// with up to 8 knots per workgroup; 512 threads when the knots need it, or when
+// only then do all workgroups fit one XCD (shared L2: XCD-local exchanges).
 // Returns GBDPCG_ERR_UNSUPPORTED when the shape does not suit this kernel.
 static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, uint32_t kcap, Layout *L,
                            int only_nt = 0, bool fused = true) {
@@ -269,6 +270,8 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
+  Layout spread;  // first layout found whose workgroups span several XCDs
+  bool have_spread = false;
   for (int nt : {256, 512}) {
     if (only_nt && nt != only_nt) continue;
     if (!fused && nt != 256) continue;  // reference-recurrence instances: 256 threads
@@ -289,16 +292,26 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
     HIPCHK(reg_occupancy(dtype, (int)n, nt, lds, &occ, fused));
     const uint32_t per_xcd = (uint32_t)(di.cus / kXcds) * (uint32_t)occ;
     if (occ <= 0 || G > (uint32_t)di.cus * (uint32_t)occ) continue;
-    L->G = G;
-    L->lds = lds;
-    L->reg = true;
-    L->fused = fused;
-    L->nt = nt;
-    L->stride = G <= per_xcd ? (uint32_t)kXcds : 1u;
-    L->grid = G * L->stride;
-    return GBDPCG_OK;
+    Layout c;
+    c.G = G;
+    c.lds = lds;
+    c.reg = true;
+    c.fused = fused;
+    c.nt = nt;
+    c.stride = G <= per_xcd ? (uint32_t)kXcds : 1u;
+    c.grid = G * c.stride;
+    if (c.stride == (uint32_t)kXcds) {
+      *L = c;
+      return GBDPCG_OK;
+    }
+    if (!have_spread) {
+      spread = c;
+      have_spread = true;
+    }
   }
-  return GBDPCG_ERR_UNSUPPORTED;
+  if (!have_spread) return GBDPCG_ERR_UNSUPPORTED;
+  *L = spread;
+  return GBDPCG_OK;
 }
 
 // Layout of a single solve: the autotuner's choice for this shape when the config
