@@ -22,7 +22,9 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HOST_FLAGS = ["--offload-arch=" + ARCH, "-O2", "-std=c++17", "-fPIC", "-Wall"]
 
-HIP_SOURCES = ["pcg_kernel.hip", "pcg_reg_kernel.hip", "pcg_reg_ref_kernel.hip", "aux_kernels.hip"]
+HIP_SOURCES = ["pcg_kernel.hip", "pcg_reg_kernel.hip", "pcg_reg_ref_kernel.hip", "pcg_rowcast_kernel.hip",
+               "pcg_pair_kernel_f32.hip", "pcg_pair_kernel_f64.hip",
+               "aux_kernels.hip"]
 CPP_SOURCES = ["api.cpp", "batch.cpp", "problem.cpp", "tune.cpp"]
 
 

@@ -209,6 +209,8 @@ struct Layout {
   uint32_t G = 0, stride = 1, grid = 0;
   size_t lds = 0;
   bool reg = false;    // register-resident kernel
+  bool bc = false;     // ... in the row-broadcast layout
+  bool pair = false;   // ... running forward and adjoint together
   bool fused = true;   // its recurrence (the LDS kernel takes it from the flags)
   int nt = 256;        // threads per workgroup
 };
@@ -265,8 +267,9 @@ static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t w
 // only then do all workgroups fit one XCD (shared L2: XCD-local exchanges).
 // Returns GBDPCG_ERR_UNSUPPORTED when the shape does not suit this kernel.
 static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, uint32_t kcap, Layout *L,
-                           int only_nt = 0, bool fused = true) {
-  if (!reg_supported((int)n) || N < 4) return GBDPCG_ERR_UNSUPPORTED;
+                           int only_nt = 0, bool fused = true, bool bc = false, bool pair = false) {
+  const RegVariant rv{fused, bc, pair && fused && n <= 14};
+  if (!reg_supported((int)n, bc) || N < 4) return GBDPCG_ERR_UNSUPPORTED;
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
@@ -275,9 +278,10 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
   for (int nt : {256, 512}) {
     if (only_nt && nt != only_nt) continue;
     if (!fused && nt != 256) continue;  // reference-recurrence instances: 256 threads
-    // FP64 rows of n = 28 need ~176 VGPRs of matrix chunks: one wave per SIMD only
-    if (dtype == GBDPCG_F64 && n > 14 && nt > 256) continue;
-    const uint32_t kmax = reg_max_knots(dtype, n, nt);
+    // FP64 rows of n = 28 (and whole FP64 rows in the row-broadcast layout) need
+    // more VGPRs than two waves per SIMD leave: one wave per SIMD only
+    if (dtype == GBDPCG_F64 && (n > 14 || bc) && nt > 256) continue;
+    const uint32_t kmax = reg_max_knots(dtype, n, nt, bc);
     if (kmax < 2) continue;
     uint32_t G = want_G;
     if (!G) {  // about 8 knots (or the cap) per workgroup, never fewer than 2
@@ -285,17 +289,19 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
       G = std::min(ceil_div(N, target), N / 2);
     }
     if (G < 1 || N / G < 2 || ceil_div(N, G) > kmax || G > (uint32_t)kMaxWgsPerSystem) continue;
-    const size_t lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
+    const size_t lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt, rv);
     if (lds > di.max_lds) continue;
-    HIPCHK(reg_prepare(dtype, (int)n, nt, lds, fused));
+    HIPCHK(reg_prepare(dtype, (int)n, nt, lds, rv));
     int occ = 0;
-    HIPCHK(reg_occupancy(dtype, (int)n, nt, lds, &occ, fused));
+    HIPCHK(reg_occupancy(dtype, (int)n, nt, lds, &occ, rv));
     const uint32_t per_xcd = (uint32_t)(di.cus / kXcds) * (uint32_t)occ;
     if (occ <= 0 || G > (uint32_t)di.cus * (uint32_t)occ) continue;
     Layout c;
     c.G = G;
     c.lds = lds;
     c.reg = true;
+    c.bc = bc;
+    c.pair = rv.pair;
     c.fused = fused;
     c.nt = nt;
     c.stride = G <= per_xcd ? (uint32_t)kXcds : 1u;
@@ -324,8 +330,8 @@ static std::string single_key(int dev, int dtype, uint32_t n, uint32_t N, bool f
 
 // Rebuilds a layout from its (kernel, threads, workgroups, stride); checks that it
 // still fits the device.
-static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint32_t G, uint32_t stride,
-                       Layout *L, bool fused) {
+static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, bool bc, int nt, uint32_t G,
+                       uint32_t stride, Layout *L, bool fused, bool pair = false) {
   DeviceInfo di;
   int rc = device_info(0, &di);
   if (rc < 0) return rc;
@@ -335,17 +341,20 @@ static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint
   t.stride = stride;
   t.grid = G * stride;
   t.reg = reg;
+  t.bc = reg && bc;
+  t.pair = reg && fused && pair && n <= 14;
   t.fused = fused;
+  const RegVariant rv{fused, t.bc, t.pair};
   t.nt = reg ? nt : kThreads;
   int occ = 0;
   if (reg) {
-    if (!reg_supported((int)n) || ceil_div(N, G) > reg_max_knots(dtype, n, nt) || N / G < 2 ||
-        (!fused && nt != 256))
+    if (!reg_supported((int)n, t.bc) || ceil_div(N, G) > reg_max_knots(dtype, n, nt, t.bc) || N / G < 2 ||
+        (!fused && nt != 256) || (t.bc && dtype == GBDPCG_F64 && nt != 256))
       return GBDPCG_ERR_UNSUPPORTED;
-    t.lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt);
+    t.lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt, rv);
     if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
-    HIPCHK(reg_prepare(dtype, (int)n, nt, t.lds, fused));
-    HIPCHK(reg_occupancy(dtype, (int)n, nt, t.lds, &occ, fused));
+    HIPCHK(reg_prepare(dtype, (int)n, nt, t.lds, rv));
+    HIPCHK(reg_occupancy(dtype, (int)n, nt, t.lds, &occ, rv));
   } else {
     t.lds = item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype));
     if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
@@ -360,16 +369,24 @@ static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, int nt, uint
 }
 
 static int choose_layout(int dev, int dtype, uint32_t n, uint32_t N, bool fused, const gbdpcg_config *cfg,
-                         Layout *L) {
+                         Layout *L, bool pair = false) {
   if (cfg->grid[0] == 0 && cfg->knots_per_wg == 0 && cfg->kernel == GBDPCG_KERNEL_AUTO) {
     TuneChoice c;
     if (tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
-        layout_from(dtype, n, N, c.a != 0, (int)c.b, c.c, c.d, L, fused) == GBDPCG_OK)
+        layout_from(dtype, n, N, c.a != 0, c.a == 2, (int)c.b, c.c, c.d, L, fused, pair) == GBDPCG_OK)
       return GBDPCG_OK;
   }
+  // automatic: the row-broadcast layout first for FP64 (measured faster there),
+  // the column-chunk register layout first for FP32
   int rc = GBDPCG_ERR_UNSUPPORTED;
-  if (cfg->kernel != GBDPCG_KERNEL_LDS)
-    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, 0, fused);
+  if ((cfg->kernel == GBDPCG_KERNEL_AUTO && dtype == GBDPCG_F64) || cfg->kernel == GBDPCG_KERNEL_ROWCAST)
+    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, 0, fused, true, pair);
+  if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_ROWCAST) {
+    set_error("row-broadcast kernel needs n in {12,14}, N >= 4 and a fitting layout");
+    return GBDPCG_ERR_UNSUPPORTED;
+  }
+  if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_REGISTER))
+    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, 0, fused, false, pair);
   if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_REGISTER) {
     set_error("register kernel needs n in {2,6,12,14,28}, N >= 4 and a fitting layout");
     return GBDPCG_ERR_UNSUPPORTED;
@@ -397,6 +414,10 @@ static int validate_cfg(const gbdpcg_config *cfg) {
   }
   if (cfg->recurrence != GBDPCG_RECUR_REFERENCE && cfg->recurrence != GBDPCG_RECUR_FUSED) {
     set_error("bad recurrence %d", cfg->recurrence);
+    return GBDPCG_ERR_INVALID;
+  }
+  if (cfg->kernel > GBDPCG_KERNEL_ROWCAST) {
+    set_error("bad kernel %u", cfg->kernel);
     return GBDPCG_ERR_INVALID;
   }
   if (cfg->precond != GBDPCG_PRECOND_JACOBI && cfg->precond != GBDPCG_PRECOND_SS) {
@@ -451,8 +472,10 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
 
   bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED;
   if (fused && N < 2) fused = false;  // one knot: the reference recurrence
+  // forward + adjoint on a register kernel (n <= 14): both solves in one loop
+  const bool pair = fused && (flags & KF_FWD) && (flags & KF_ADJ) && n <= 14;
   Layout L;
-  rc = choose_layout(dev, dtype, n, N, fused, cfg, &L);
+  rc = choose_layout(dev, dtype, n, N, fused, cfg, &L, pair);
   if (rc < 0) return rc;
 
   if (Pinv == nullptr) {
@@ -487,7 +510,8 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
   one.flags = flags | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
               exchange_flags();
   if (L.reg) {
-    HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream, fused));
+    HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream,
+                      RegVariant{fused, L.bc, L.pair}));
   } else {
     HIPCHK(pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
   }
@@ -682,30 +706,34 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   std::vector<Layout> cands;
   auto add = [&](const Layout &L) {
     for (const Layout &c : cands)
-      if (c.reg == L.reg && c.nt == L.nt && c.G == L.G && c.stride == L.stride) return;
+      if (c.reg == L.reg && c.bc == L.bc && c.nt == L.nt && c.G == L.G && c.stride == L.stride) return;
     cands.push_back(L);
   };
   {
     Layout def;
     rc = GBDPCG_ERR_UNSUPPORTED;
-    if (cfg->kernel != GBDPCG_KERNEL_LDS) rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused);
-    if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel != GBDPCG_KERNEL_REGISTER)
+    if ((cfg->kernel == GBDPCG_KERNEL_AUTO && dtype == GBDPCG_F64) || cfg->kernel == GBDPCG_KERNEL_ROWCAST)
+      rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused, true);
+    if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_REGISTER))
+      rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused);
+    if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_LDS))
       rc = plan_single(dtype, n, N, fused, 0, 0, &def);
     if (rc < 0) return rc;
     add(def);
   }
   const uint32_t Ks[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64};
-  for (int reg = 1; reg >= 0; --reg) {
-    if (reg && cfg->kernel == GBDPCG_KERNEL_LDS) continue;
-    if (!reg && cfg->kernel == GBDPCG_KERNEL_REGISTER) continue;
+  // kind 2: row-broadcast register layout, 1: register, 0: LDS
+  for (int kind = 2; kind >= 0; --kind) {
+    const uint32_t kid = kind == 2 ? GBDPCG_KERNEL_ROWCAST : kind == 1 ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+    if (cfg->kernel != GBDPCG_KERNEL_AUTO && cfg->kernel != kid) continue;
     for (int nt : {256, 512}) {
-      if (!reg && nt != kThreads) continue;
+      if (!kind && nt != kThreads) continue;
       for (uint32_t K : Ks) {
         const uint32_t G = ceil_div(N, K);
         if (G > (uint32_t)kMaxWgsPerSystem || (fused && N / G < 2)) continue;
         for (uint32_t stride : {(uint32_t)kXcds, 1u}) {
           Layout L;
-          if (layout_from(dtype, n, N, reg != 0, nt, G, stride, &L, fused) == GBDPCG_OK) add(L);
+          if (layout_from(dtype, n, N, kind != 0, kind == 2, nt, G, stride, &L, fused) == GBDPCG_OK) add(L);
         }
       }
     }
@@ -775,7 +803,8 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     for (uint32_t t = 0; t <= trials; ++t) {
       if ((e = hipMemsetAsync(lam, 0, (size_t)n * N * ts, st)) != hipSuccess) return fail("hipMemsetAsync", e);
       if ((e = hipEventRecord(ea, st)) != hipSuccess) return fail("hipEventRecord", e);
-      e = L.reg ? reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st, fused)
+      e = L.reg ? reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st,
+                             RegVariant{fused, L.bc, false})
                 : pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, st);
       if (e != hipSuccess) return fail("solve launch", e);
       if ((e = hipEventRecord(eb, st)) != hipSuccess) return fail("hipEventRecord", e);
@@ -810,7 +839,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   }
   const Layout &B = cands[best];
   tune_store(single_key(dev, dtype, n, N, fused, cfg->exit_mode),
-             TuneChoice{B.reg ? 1u : 0u, (uint32_t)B.nt, B.G, B.stride});
+             TuneChoice{B.bc ? 2u : B.reg ? 1u : 0u, (uint32_t)B.nt, B.G, B.stride});
   if (report) {
     gbdpcg_tune_report r{};
     r.candidates = timed;
@@ -818,7 +847,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     r.knots_per_wg = ceil_div(N, B.G);
     r.threads = (uint32_t)B.nt;
     r.wgs = B.G;
-    r.kernel = B.reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+    r.kernel = B.bc ? GBDPCG_KERNEL_ROWCAST : B.reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
     r.best_us = best_us;
     r.default_us = default_us;
     *report = r;

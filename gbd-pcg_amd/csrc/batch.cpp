@@ -48,12 +48,18 @@ struct gbdpcg_batch {
   gbdpcg_batch_info info{};
   size_t slot_words = 0;
   bool reg = false;  // register-resident kernel
+  bool bc = false;   // ... in the row-broadcast layout
+  bool pair = false; // ... with forward + adjoint systems iterated together
   int kn = 0;        // its stateSize specialisation (0 = runtime dispatch)
   int nt = 256;      // threads per workgroup
 };
 
 namespace gbdpcg {
 bool batch_uses_registers(const gbdpcg_batch *plan) { return plan && plan->reg; }
+int batch_kernel(const gbdpcg_batch *plan) {
+  if (!plan) return GBDPCG_KERNEL_AUTO;
+  return plan->bc ? GBDPCG_KERNEL_ROWCAST : plan->reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+}
 int batch_threads(const gbdpcg_batch *plan) { return plan ? plan->nt : 0; }
 }  // namespace gbdpcg
 
@@ -79,7 +85,8 @@ uint32_t max_knots(uint32_t n, uint32_t N, size_t ts, size_t budget) {
 
 // Planner's cost of one system (ns, rough): iterations x (exchanges + on-chip work)
 // plus the HBM load of its block rows.
-double system_cost(const gbdpcg_system_desc &d, uint32_t G, bool fused, size_t ts) {
+// pair: forward + adjoint iterated together (one loop, both matvecs per iteration).
+double system_cost(const gbdpcg_system_desc &d, uint32_t G, bool fused, size_t ts, bool pair) {
   const double K = std::ceil((double)d.N / G);
   const double nP = (double)((d.n + 3) & ~3u);
   const double iters = std::min<double>(d.max_iter, 40.0);
@@ -87,6 +94,7 @@ double system_cost(const gbdpcg_system_desc &d, uint32_t G, bool fused, size_t t
   const double exch = (fused ? 1.0 : 2.0) * (G <= 1 ? 250.0 : 900.0 + 12.0 * G);
   const double mv = 2.0 * K * d.n * 3.0 * nP / 32.0 / 2.4 * (ts / 4.0);
   const double load = 2.0 * 3.0 * d.n * d.n * K * ts / 50.0;
+  if (pair && passes == 2) return iters * (exch + 2 * mv) + 3 * exch + load + 3000.0;
   return passes * (iters * (exch + mv) + 2 * exch) + load + 3000.0;
 }
 
@@ -138,7 +146,27 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28} and N >= 4 for every system");
     return GBDPCG_ERR_UNSUPPORTED;
   }
-  const int nt = (use_reg && same_n && (cfg->block[0] == 256 || f64_wide)) ? 256 : 512;
+  // row-broadcast layout (on request; the autotuner tries it): one stateSize in
+  // {12, 14} for every system
+  const bool bc = use_reg && same_n && reg_supported((int)systems[0].n, true) &&
+                  cfg->kernel == GBDPCG_KERNEL_ROWCAST;
+  if (cfg->kernel == GBDPCG_KERNEL_ROWCAST && !bc) {
+    set_error("row-broadcast kernel needs the fused recurrence, one n in {12,14} and N >= 4 for every system");
+    return GBDPCG_ERR_UNSUPPORTED;
+  }
+  // forward + adjoint systems run both solves in one loop (the paired kernel;
+  // other systems of the plan run one pass after the other on it as usual)
+  // on rows of n <= 14
+  bool any_pair = false, small_n = true;
+  for (uint32_t i = 0; i < count; ++i) {
+    if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) any_pair = true;
+    if (systems[i].n > 14) small_n = false;
+  }
+  const bool pair = use_reg && any_pair && small_n;
+  const RegVariant rv{true, bc, pair};
+  // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads only
+  const bool only256 = f64_wide || (bc && dtype == GBDPCG_F64);
+  const int nt = (use_reg && same_n && (cfg->block[0] == 256 || only256)) ? 256 : 512;
   const int kn = same_n ? (int)systems[0].n : 0;
 
   // workgroups per system and the launch's LDS size
@@ -149,7 +177,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   const uint32_t kcap = cfg->knots_per_wg ? std::max<uint32_t>(cfg->knots_per_wg, 1u) : 0u;
   for (uint32_t i = 0; use_reg && i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
-    const uint32_t kmax = reg_max_knots(dtype, d.n, nt);
+    const uint32_t kmax = reg_max_knots(dtype, d.n, nt, bc);
     const uint32_t target = kcap ? std::min(kmax, std::max<uint32_t>(kcap, 2u)) : kmax;
     uint32_t g = d.wgs;
     if (!g) {  // the cap is a target: every workgroup keeps at least 2 knots
@@ -163,7 +191,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     }
     G[i] = g;
     fused[i] = 1;
-    lds = std::max(lds, reg_lds_bytes(dtype, d.n, (d.N + g - 1) / g, nt));
+    lds = std::max(lds, reg_lds_bytes(dtype, d.n, (d.N + g - 1) / g, nt, rv));
   }
   for (uint32_t i = 0; !use_reg && i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
@@ -189,8 +217,8 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   lds = (lds + 255) & ~(size_t)255;
   int occ = 0;
   if (use_reg) {
-    HIPCHK(reg_prepare(dtype, kn, nt, lds));
-    HIPCHK(reg_occupancy(dtype, kn, nt, lds, &occ));
+    HIPCHK(reg_prepare(dtype, kn, nt, lds, rv));
+    HIPCHK(reg_occupancy(dtype, kn, nt, lds, &occ, rv));
   } else {
     HIPCHK(pcg_prepare(dtype, lds));
     HIPCHK(pcg_occupancy(dtype, lds, &occ));
@@ -210,7 +238,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
 
   // gang list scheduling
   std::vector<double> cost(count);
-  for (uint32_t i = 0; i < count; ++i) cost[i] = system_cost(systems[i], G[i], fused[i], ts);
+  for (uint32_t i = 0; i < count; ++i) cost[i] = system_cost(systems[i], G[i], fused[i], ts, pair);
   std::vector<uint32_t> order(count);
   std::iota(order.begin(), order.end(), 0u);
   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
@@ -333,6 +361,8 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   plan->info.wgs_per_cu = (uint32_t)occ;
   plan->info.lds_bytes = (uint32_t)lds;
   plan->reg = use_reg;
+  plan->bc = bc;
+  plan->pair = pair;
   plan->kn = kn;
   plan->nt = use_reg ? nt : kThreads;
   plan->info.systems = count;
@@ -351,7 +381,8 @@ int gbdpcg_batch_run(gbdpcg_batch *plan, void *stream) {
   memset(&dummy, 0, sizeof dummy);
   if (plan->reg) {
     HIPCHK(reg_launch(plan->dtype, plan->kn, plan->nt, dummy, plan->d_sys, plan->d_items, plan->info.rounds, 1,
-                      plan->info.grid, plan->info.lds_bytes, static_cast<hipStream_t>(stream)));
+                      plan->info.grid, plan->info.lds_bytes, static_cast<hipStream_t>(stream),
+                      RegVariant{true, plan->bc, plan->pair}));
     return GBDPCG_OK;
   }
   HIPCHK(pcg_launch(plan->dtype, dummy, plan->d_sys, plan->d_items, plan->info.rounds, 1, plan->info.grid,

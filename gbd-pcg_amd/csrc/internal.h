@@ -69,11 +69,13 @@ __host__ __device__ constexpr uint32_t reg_chunk(uint32_t n, uint32_t lpr) {
 }
 
 // Words (64-bit) of one exchange record: kRecScalars scalar slots (3 per wave of
-// the publishing workgroup, up to 8 waves) + left edge (2 knots of n) + right
-// edge (2 knots of n), each value split into 32-bit payload words.
+// the publishing workgroup, up to 8 waves) + two edge blocks, each a left edge
+// (2 knots of n) + a right edge (2 knots of n); single solves use the first
+// block, paired forward + adjoint solves both.  Each value is split into 32-bit
+// payload words.
 constexpr uint32_t kRecScalars = 24;
 __host__ __device__ constexpr uint32_t rec_words(uint32_t n, uint32_t word_per_value) {
-  return (kRecScalars + 4u * n) * word_per_value;
+  return (kRecScalars + 8u * n) * word_per_value;
 }
 
 // Number of T elements of the dynamic LDS used by one workgroup owning K knots.
@@ -141,16 +143,23 @@ hipError_t warm_start_launch(int dtype, uint32_t n, uint32_t N, void *lambda, vo
                              hipStream_t stream);
 hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t count, uint64_t seed, double scale,
                                    hipStream_t stream);
-// pcg_reg_kernel.hip (register-resident latency kernel, fused recurrence)
-bool reg_supported(int n);
-size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt);
-uint32_t reg_max_knots(int dtype, uint32_t n, int nt);
-// fused = false selects the reference-recurrence instances (256 threads only)
-hipError_t reg_prepare(int dtype, int n, int nt, size_t lds, bool fused = true);
-hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu, bool fused = true);
+// pcg_reg_kernel.hip (register-resident latency kernel) and its variants.
+struct RegVariant {
+  bool fused = true;  // fused recurrence; false: reference recurrence (256 threads only)
+  bool bc = false;    // row-broadcast layout (pcg_rowcast_kernel.hip, n in {12, 14})
+  bool pair = false;  // forward + adjoint items iterated together (pcg_pair_kernel_*.hip)
+};
+bool reg_supported(int n, bool bc = false);
+size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt, RegVariant v = {});
+uint32_t reg_max_knots(int dtype, uint32_t n, int nt, bool bc = false);
+hipError_t reg_prepare(int dtype, int n, int nt, size_t lds, RegVariant v = {});
+hipError_t reg_occupancy(int dtype, int n, int nt, size_t lds, int *wgs_per_cu, RegVariant v = {});
 hipError_t reg_launch(int dtype, int n, int nt, const KSys &one, const KSys *systems, const uint32_t *items,
                       uint32_t rounds, uint32_t stride, uint32_t grid, size_t lds, hipStream_t stream,
-                      bool fused = true);
+                      RegVariant v = {});
+const void *rowcast_kernel_ptr(int dtype, int n, int nt, bool fused);
+const void *pair_kernel_ptr_f32(int n, int nt, bool bc);
+const void *pair_kernel_ptr_f64(int n, int nt, bool bc);
 // pcg_reg_ref_kernel.hip: reference-recurrence kernel for (dtype, n, threads), or null
 const void *reg_ref_kernel_ptr(int dtype, int n, int nt);
 // tune.cpp: autotuner choice cache (a..d: meaning fixed by the key's kind)
@@ -172,5 +181,6 @@ namespace gbdpcg {
 uint32_t exchange_flags();
 // batch.cpp: properties of a built plan used by the tuner
 bool batch_uses_registers(const gbdpcg_batch *plan);
+int batch_kernel(const gbdpcg_batch *plan);  // GBDPCG_KERNEL_LDS / _REGISTER / _ROWCAST
 int batch_threads(const gbdpcg_batch *plan);
 }  // namespace gbdpcg

@@ -46,12 +46,14 @@ struct RegOut {
 };
 
 // Status of one exchange within the item (all threads agree after a gather).
-template <typename T, int NS, int NT>
+// BC: row-broadcast layout (one lane per matrix row, 16 lanes per knot, every
+// lane holding its whole row of 3n columns).
+template <typename T, int NS, int NT, bool BC = false>
 struct RegItem {
   static constexpr int n = NS;
   static constexpr int nP = (int)padded_n(NS);
-  static constexpr int LPR = (int)reg_lpr(NS, NT, (int)sizeof(T));  // lanes per matrix row
-  static constexpr int CW = (int)reg_chunk(NS, LPR);                 // columns per lane
+  static constexpr int LPR = BC ? 1 : (int)reg_lpr(NS, NT, (int)sizeof(T));  // lanes per matrix row
+  static constexpr int CW = BC ? 3 * NS : (int)reg_chunk(NS, LPR);           // columns per lane
   static constexpr int WPV = (int)(sizeof(T) / 4);
   static constexpr int RW = (int)rec_words(NS, WPV);
   static constexpr int NWAVES = NT / 64;
@@ -93,33 +95,35 @@ struct RegItem {
     if (e < 2) slot_hi = (kRecScalars + 2 * n + e * n + row) | (e << 16);
   }
 
+  // voff: 0 for the first edge block of a record, 4n for the second (paired
+  // solves publish the edges of two vectors per exchange).
   template <bool XCD>
-  __device__ __forceinline__ void store_edges(uint64_t *r, T val, int edge, uint32_t epoch) const {
+  __device__ __forceinline__ void store_edges(uint64_t *r, T val, int edge, uint32_t epoch, int voff) const {
     const int lim = edge << 16;
     if (slot_lo >= 0 && slot_lo < lim) {
 #pragma unroll
       for (int qq = 0; qq < WPV; ++qq) {
-        uint64_t *p = r + (slot_lo & 0xFFFF) * WPV + qq;
+        uint64_t *p = r + ((slot_lo & 0xFFFF) + voff) * WPV + qq;
         if constexpr (XCD) xstore_xcd(p, payload_word(val, qq), epoch); else xstore(p, payload_word(val, qq), epoch);
       }
     }
     if (slot_hi >= 0 && slot_hi < lim) {
 #pragma unroll
       for (int qq = 0; qq < WPV; ++qq) {
-        uint64_t *p = r + (slot_hi & 0xFFFF) * WPV + qq;
+        uint64_t *p = r + ((slot_hi & 0xFFFF) + voff) * WPV + qq;
         if constexpr (XCD) xstore_xcd(p, payload_word(val, qq), epoch); else xstore(p, payload_word(val, qq), epoch);
       }
     }
   }
 
-  __device__ __forceinline__ void publish_edge(T val, int edge) const {
+  __device__ __forceinline__ void publish_edge(T val, int edge, int voff = 0) const {
     if (G == 1) return;
     const uint32_t epoch = base + seq;
     uint64_t *r = rec((int)(seq & 1u), rank);
     if (xcd)
-      store_edges<true>(r, val, edge, epoch);
+      store_edges<true>(r, val, edge, epoch, voff);
     else
-      store_edges<false>(r, val, edge, epoch);
+      store_edges<false>(r, val, edge, epoch, voff);
   }
 
   // Workgroup totals of three per-lane partials (DPP per wave, then the waves in
@@ -240,6 +244,116 @@ struct RegItem {
     t2 = uniform(tot[2]);
     return __builtin_amdgcn_readfirstlane(ctl[0]) == 0;
   }
+
+  // Paired solves: six workgroup totals (the first three for vector A, the last
+  // three for B; each scalar reduced exactly as reduce_publish reduces it),
+  // wsum holding 6 per wave.  Lanes 0..5 publish.  placement as in
+  // reduce_publish, for scalars 1 and 2.
+  __device__ __forceinline__ void reduce_publish6(T s0, T s1, T s2, T s3, T s4, T s5,
+                                                  bool placement = false) const {
+    wave_sum3(s0, s1, s2);
+    wave_sum3(s3, s4, s5);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+      wsum[w * 6 + 0] = s0;
+      wsum[w * 6 + 1] = s1;
+      wsum[w * 6 + 2] = s2;
+      wsum[w * 6 + 3] = s3;
+      wsum[w * 6 + 4] = s4;
+      wsum[w * 6 + 5] = s5;
+    }
+    __syncthreads();
+    const int l = threadIdx.x;
+    if (l >= 6) return;
+    T tv = wsum[l];
+#pragma unroll
+    for (int i = 1; i < NWAVES; ++i) tv += wsum[i * 6 + l];
+    if (G == 1) {
+      tot[l] = tv;
+      return;
+    }
+    if (placement && (l == 1 || l == 2)) {
+      const T x = (T)(1 + xcc_id());
+      tv = l == 1 ? x : x * x;
+    }
+    const uint32_t epoch = base + seq;
+    uint64_t *r = rec((int)(seq & 1u), rank);
+#pragma unroll
+    for (int qq = 0; qq < WPV; ++qq) xstore_sel(r + l * WPV + qq, payload_word(tv, qq), epoch, xcd);
+  }
+
+  // gather for paired solves: six totals, and the neighbours' `edge` edge knots
+  // of vA (first edge block) and vB (second) into their ghost knots.
+  __device__ __forceinline__ bool gather_pair(T *vA, T *vB, int edge, T (&tt)[6]) {
+    if (G == 1) {
+      __syncthreads();
+#pragma unroll
+      for (int l = 0; l < 6; ++l) tt[l] = uniform(tot[l]);
+      ++seq;
+      return true;
+    }
+    const uint32_t epoch = base + seq;
+    const int type = (int)(seq & 1u);
+    bool ok = true;
+    const int t = opaque_tid();
+    if (t < 64) {
+      constexpr int NW = 6 * WPV;
+      T acc[6] = {0, 0, 0, 0, 0, 0};
+      for (int g = t; g < G; g += 64) {
+        uint32_t wv_[NW];
+        if (!xwait_words<NW>(rec(type, g), epoch, wv_)) {
+          ok = false;
+          break;
+        }
+#pragma unroll
+        for (int l = 0; l < 6; ++l) acc[l] += word_value(wv_, l);
+      }
+#pragma unroll
+      for (int l = 0; l < 6; ++l) acc[l] = wave_sum(acc[l]);
+      if (t == 0) {
+#pragma unroll
+        for (int l = 0; l < 6; ++l) tot[l] = acc[l];
+      }
+    } else {
+      const int en = edge * n * WPV;
+      const int nleft = rank > 0 ? en : 0;
+      const int nright = rank < G - 1 ? en : 0;
+      const int per = nleft + nright;
+      for (int i = t - 64; i < 2 * per; i += NT - 64) {
+        const int v = i >= per ? 1 : 0;
+        const int iv = i - v * per;
+        const int voff = v * 4 * n;
+        T *vec = v ? vB : vA;
+        uint64_t *src;
+        uint32_t *dst;
+        if (iv < nleft) {  // left neighbour's right edge: knot a-1-e = local 1-e
+          const int vi = iv / WPV, qq = iv - vi * WPV;
+          const int e = vi / n, rr = vi - e * n;
+          src = rec(type, rank - 1) + (kRecScalars + voff + 2 * n + e * n + rr) * WPV + qq;
+          dst = reinterpret_cast<uint32_t *>(vec + (1 - e) * nP + rr) + qq;
+        } else {  // right neighbour's left edge: knot b+e = local K+2+e
+          const int ii = iv - nleft;
+          const int vi = ii / WPV, qq = ii - vi * WPV;
+          const int e = vi / n, rr = vi - e * n;
+          src = rec(type, rank + 1) + (kRecScalars + voff + e * n + rr) * WPV + qq;
+          dst = reinterpret_cast<uint32_t *>(vec + (K + 2 + e) * nP + rr) + qq;
+        }
+        uint32_t val = 0;
+        if (xcd ? xwait(src, epoch, val) : xwait_lazy(src, epoch, val)) {
+          *dst = val;
+        } else {
+          ok = false;
+          break;
+        }
+      }
+    }
+    if (!ok) ctl[0] = 1;
+    __syncthreads();
+    ++seq;
+#pragma unroll
+    for (int l = 0; l < 6; ++l) tt[l] = uniform(tot[l]);
+    return __builtin_amdgcn_readfirstlane(ctl[0]) == 0;
+  }
 };
 
 // Four accumulator chains (the FMA latency, not the issue rate, bounds a short
@@ -299,14 +413,109 @@ __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A
   }
 }
 
+// Row-broadcast layout: the whole row `row` of block row kg (3n values, block
+// column s, column k at index s*n + k).
+template <typename T, int NS>
+__device__ __forceinline__ void load_row(T (&dst)[3 * NS], const T *__restrict__ A, int kg, int row, int N,
+                                         bool active) {
+  constexpr int n = NS;
+  int rl = row;
+  asm volatile("" : "+v"(rl));
+#pragma unroll
+  for (int j = 0; j < 3 * n; ++j) {
+    const int seg = j / n;
+    T v = 0;
+    if (active && kg >= 0 && kg < N && !(kg == 0 && seg == 0) && !(kg == N - 1 && seg == 2))
+      v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)j * n + rl];
+    dst[j] = v;
+  }
+}
+
+// a0/a1 += m[k] * (x of lane k of this lane's 16-lane row), k = 0..NS-1, on
+// alternate accumulators.  DPP row_newbcast:k hands every lane of a row the
+// value of its lane k, so one VALU op does one multiply-add with no LDS traffic.
+// The compiler does not fold a broadcast into an accumulating FMA by itself,
+// hence the asm; each block opens with s_nop 4, the 5 wait states a DPP read
+// needs after a VALU write of EXEC (2 after a VALU write of its source VGPR).
+#define GBDPCG_RC(OP, k, a, mi) OP " %" #a ", %2, %" #mi " row_newbcast:" #k " row_mask:0xf bank_mask:0xf\n\t"
+#define GBDPCG_RC12(OP)                                                                                     \
+  GBDPCG_RC(OP, 0, 0, 3) GBDPCG_RC(OP, 1, 1, 4) GBDPCG_RC(OP, 2, 0, 5) GBDPCG_RC(OP, 3, 1, 6)                \
+  GBDPCG_RC(OP, 4, 0, 7) GBDPCG_RC(OP, 5, 1, 8) GBDPCG_RC(OP, 6, 0, 9) GBDPCG_RC(OP, 7, 1, 10)               \
+  GBDPCG_RC(OP, 8, 0, 11) GBDPCG_RC(OP, 9, 1, 12) GBDPCG_RC(OP, 10, 0, 13) GBDPCG_RC(OP, 11, 1, 14)
+#define GBDPCG_RC14(OP) GBDPCG_RC12(OP) GBDPCG_RC(OP, 12, 0, 15) GBDPCG_RC(OP, 13, 1, 16)
+#define GBDPCG_M12(m) "v"(m[0]), "v"(m[1]), "v"(m[2]), "v"(m[3]), "v"(m[4]), "v"(m[5]), "v"(m[6]), \
+    "v"(m[7]), "v"(m[8]), "v"(m[9]), "v"(m[10]), "v"(m[11])
+#define GBDPCG_M14(m) GBDPCG_M12(m), "v"(m[12]), "v"(m[13])
+
+template <typename T, int NS>
+struct RowCast;
+template <>
+struct RowCast<float, 14> {
+  static __device__ __forceinline__ void fma(float &a0, float &a1, float x, const float *m) {
+    asm("s_nop 4\n\t" GBDPCG_RC14("v_fmac_f32_dpp") : "+v"(a0), "+v"(a1) : "v"(x), GBDPCG_M14(m));
+  }
+};
+template <>
+struct RowCast<float, 12> {
+  static __device__ __forceinline__ void fma(float &a0, float &a1, float x, const float *m) {
+    asm("s_nop 4\n\t" GBDPCG_RC12("v_fmac_f32_dpp") : "+v"(a0), "+v"(a1) : "v"(x), GBDPCG_M12(m));
+  }
+};
+template <>
+struct RowCast<double, 14> {
+  static __device__ __forceinline__ void fma(double &a0, double &a1, double x, const double *m) {
+    asm("s_nop 4\n\t" GBDPCG_RC14("v_fmac_f64_dpp") : "+v"(a0), "+v"(a1) : "v"(x), GBDPCG_M14(m));
+  }
+};
+template <>
+struct RowCast<double, 12> {
+  static __device__ __forceinline__ void fma(double &a0, double &a1, double x, const double *m) {
+    asm("s_nop 4\n\t" GBDPCG_RC12("v_fmac_f64_dpp") : "+v"(a0), "+v"(a1) : "v"(x), GBDPCG_M12(m));
+  }
+};
+#undef GBDPCG_RC
+#undef GBDPCG_RC12
+#undef GBDPCG_RC14
+#undef GBDPCG_M12
+#undef GBDPCG_M14
+
+// y = row . (xa | xb | xc), x* the lane's entries of three consecutive knots;
+// four accumulator chains summed in a fixed order.
+template <typename T, int NS>
+__device__ __forceinline__ T rowcast_dot(const T (&mat)[3 * NS], T xa, T xb, T xc) {
+  T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  RowCast<T, NS>::fma(a0, a1, xa, mat);
+  RowCast<T, NS>::fma(a2, a3, xb, mat + NS);
+  RowCast<T, NS>::fma(a0, a1, xc, mat + 2 * NS);
+  return (a0 + a1) + (a2 + a3);
+}
+
 __device__ __forceinline__ bool finite_d(double v) { return __builtin_isfinite(v); }
 __device__ __forceinline__ bool finite_t(float v) { return __builtin_isfinite(v); }
 __device__ __forceinline__ bool finite_t(double v) { return __builtin_isfinite(v); }
 
 
-template <typename T, int NS, int NT, bool FUSED>
+// One matvec row pass: this lane's row of a resident block row times the vector
+// window of knots m-1..m+1 (local knots m..m+2) of `vec`; 0 on inactive lanes.
+template <typename T, int NS, int NT, bool BC, int CW>
+__device__ __forceinline__ T row_mv(const RegItem<T, NS, NT, BC> &it, const T (&mat)[CW], const T *vec, bool act) {
+  using R = RegItem<T, NS, NT, BC>;
+  if constexpr (BC) {
+    // every lane of the 16-lane row takes part in the broadcasts; lanes past the
+    // window read (and discard) the last knot's entries
+    const int mm = it.m < it.K + 1 ? it.m : it.K + 1;
+    const T *v = vec + mm * R::nP + it.row;
+    const T y = rowcast_dot<T, NS>(mat, v[0], v[R::nP], v[2 * R::nP]);
+    return act ? y : (T)0;
+  } else {
+    T y = act ? chunk_dot<T, CW>(mat, vec + it.m * R::nP + it.c * CW) : (T)0;
+    return group_sum<R::LPR>(y);
+  }
+}
+
+template <typename T, int NS, int NT, bool FUSED, bool BC = false>
 __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
-  using R = RegItem<T, NS, NT>;
+  using R = RegItem<T, NS, NT, BC>;
   constexpr int n = R::n, nP = R::nP, CW = R::CW;
   R it;
   it.N = (int)d.N;
@@ -316,7 +525,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   it.b = (int)knot_begin(rank + 1, d.G, d.N);
   it.K = it.b - it.a;
   const int K = it.K;
-  it.VL = (K + 4) * nP + R::LPR * CW;
+  it.VL = (K + 4) * nP + (BC ? 16 : R::LPR * CW);
   it.slots = d.slots;
   it.tol = d.tol;
   it.rel = (d.flags & KF_REL) != 0;
@@ -327,11 +536,16 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   const int t = opaque_tid();
   it.q = t / R::LPR;
   it.c = t % R::LPR;
-  it.m = it.q / n;
-  it.row = it.q - it.m * n;
+  if constexpr (BC) {
+    it.m = t >> 4;
+    it.row = t & 15;
+  } else {
+    it.m = it.q / n;
+    it.row = it.q - it.m * n;
+  }
   it.kg = it.a - 1 + it.m;
-  it.act_u = it.m < K + 2;
-  it.act_w = it.m >= 1 && it.m <= K;
+  it.act_u = it.m < K + 2 && (!BC || it.row < n);
+  it.act_w = it.m >= 1 && it.m <= K && (!BC || it.row < n);
   it.owner = it.act_w && it.c == 0;
   it.ghost = it.c == R::LPR - 1 && it.q < 4 * n;
   it.set_edge_slots();
@@ -366,8 +580,13 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
 
   // matrix chunks straight from HBM into registers
   T mM[CW], mS[CW];
-  load_chunk<T, NS, CW>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
-  load_chunk<T, NS, CW>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+  if constexpr (BC) {
+    load_row<T, NS>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
+    load_row<T, NS>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
+  } else {
+    load_chunk<T, NS, CW>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+  }
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
   it.seq = 0;
@@ -408,8 +627,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
     // r = rhs - S x (own rows)
     T r_i;
     {
-      T y = it.act_w ? chunk_dot<T, CW>(mS, it.xv + m * nP + c * CW) : (T)0;
-      y = group_sum<R::LPR>(y);
+      T y = row_mv(it, mS, it.xv, it.act_w);
       r_i = b_i - y;
     }
     if (it.owner) it.rv[own_pos] = r_i;
@@ -435,8 +653,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       // p = z, eta = r'z
       T z_i, p_i, eta, rr;
       {
-        T y = it.act_u ? chunk_dot<T, CW>(mM, it.rv + m * nP + c * CW) : (T)0;
-        z_i = group_sum<R::LPR>(y);
+        z_i = row_mv(it, mM, it.rv, it.act_u);
         p_i = z_i;
         if (it.act_u && c == 0) it.uv[(m + 1) * nP + row] = p_i;  // uv holds p
         const T rz = it.owner ? r_i * z_i : (T)0;
@@ -472,8 +689,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
           // two edge knots of v each side: the ghost lanes' r needs them.
           T v_i;
           {
-            T y = it.act_w ? chunk_dot<T, CW>(mS, it.uv + m * nP + c * CW) : (T)0;
-            v_i = group_sum<R::LPR>(y);
+            v_i = row_mv(it, mS, it.uv, it.act_w);
           }
           it.publish_edge(v_i, 2);
           it.reduce_publish(it.owner ? p_i * v_i : (T)0, (T)0, (T)0);
@@ -502,8 +718,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
           __syncthreads();
           // z = Pinv r (own + ghost rows) ; eta' = r'z ; exit test
           {
-            T y = it.act_u ? chunk_dot<T, CW>(mM, it.rv + m * nP + c * CW) : (T)0;
-            z_i = group_sum<R::LPR>(y);
+            z_i = row_mv(it, mM, it.rv, it.act_u);
           }
           const T rz = it.owner ? r_i * z_i : (T)0;
           const T r2 = it.owner ? r_i * r_i : (T)0;
@@ -540,12 +755,10 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       T p_i, s_i, s_g;
       T eta, alpha, rr;
       {
-        T y = it.act_u ? chunk_dot<T, CW>(mM, it.rv + m * nP + c * CW) : (T)0;
-        u_i = group_sum<R::LPR>(y);
+        u_i = row_mv(it, mM, it.rv, it.act_u);
         if (it.act_u && c == 0) it.uv[(m + 1) * nP + row] = u_i;
         __syncthreads();
-        T z = it.act_w ? chunk_dot<T, CW>(mS, it.uv + m * nP + c * CW) : (T)0;
-        w_i = group_sum<R::LPR>(z);
+        w_i = row_mv(it, mS, it.uv, it.act_w);
         it.publish_edge(w_i, 2);
         const T rz = it.owner ? r_i * u_i : (T)0;
         const T wu = it.owner ? w_i * u_i : (T)0;
@@ -595,12 +808,10 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
             it.rv[it.gpos] = r_g;
           }
           __syncthreads();
-          T y = it.act_u ? chunk_dot<T, CW>(mM, it.rv + m * nP + c * CW) : (T)0;
-          u_i = group_sum<R::LPR>(y);
+          u_i = row_mv(it, mM, it.rv, it.act_u);
           if (it.act_u && c == 0) it.uv[(m + 1) * nP + row] = u_i;
           __syncthreads();
-          T z = it.act_w ? chunk_dot<T, CW>(mS, it.uv + m * nP + c * CW) : (T)0;
-          w_i = group_sum<R::LPR>(z);
+          w_i = row_mv(it, mS, it.uv, it.act_w);
           it.publish_edge(w_i, 2);
           const T rz = it.owner ? r_i * u_i : (T)0;
           const T wu = it.owner ? w_i * u_i : (T)0;
@@ -701,14 +912,360 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
     __hip_atomic_fetch_or(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Forward and adjoint solves of one system iterated together (fused recurrence;
+// both flags set).  The two solves share S and Pinv and are independent until
+// g_S is formed, so each iteration runs the two right-hand sides (A: gamma,
+// from lambda0; B: g_lambda, from 0) through the same resident matrices and one
+// exchange carrying both sets of scalars and edges.  Each side performs exactly
+// the operations of its own solve in reg_item, in the same order, and stops on
+// its own exit test; once one side has stopped, the other runs alone (its
+// matvecs and updates are skipped, the exchange continues for the other).
+template <typename T, int NS, int NT, bool BC>
+__device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *lds) {
+  using R = RegItem<T, NS, NT, BC>;
+  constexpr int n = R::n, nP = R::nP, CW = R::CW;
+  R it;
+  it.N = (int)d.N;
+  it.G = (int)d.G;
+  it.rank = (int)rank;
+  it.a = (int)knot_begin(rank, d.G, d.N);
+  it.b = (int)knot_begin(rank + 1, d.G, d.N);
+  it.K = it.b - it.a;
+  const int K = it.K;
+  it.VL = (K + 4) * nP + (BC ? 16 : R::LPR * CW);
+  it.slots = d.slots;
+  it.tol = d.tol;
+  it.rel = (d.flags & KF_REL) != 0;
+  it.xcd = false;
+  it.xcd_ok = (d.flags & KF_NOXCD) == 0;
+  const int t = opaque_tid();
+  it.q = t / R::LPR;
+  it.c = t % R::LPR;
+  if constexpr (BC) {
+    it.m = t >> 4;
+    it.row = t & 15;
+  } else {
+    it.m = it.q / n;
+    it.row = it.q - it.m * n;
+  }
+  it.kg = it.a - 1 + it.m;
+  it.act_u = it.m < K + 2 && (!BC || it.row < n);
+  it.act_w = it.m >= 1 && it.m <= K && (!BC || it.row < n);
+  it.owner = it.act_w && it.c == 0;
+  it.ghost = it.c == R::LPR - 1 && it.q < 4 * n;
+  it.set_edge_slots();
+  {
+    const int gk = it.q / n, gr = it.q - (it.q / n) * n;
+    const int gj = gk < 2 ? gk : K + gk;
+    it.gpos = gj * nP + gr;
+  }
+  // LDS: r, u, w of each side, lambda and g_gamma windows, 6 partials per wave
+  T *rA = lds, *uA = rA + it.VL, *wA = uA + it.VL;
+  T *rB = wA + it.VL, *uB = rB + it.VL, *wB = uB + it.VL;
+  it.xv = wB + it.VL;  // lambda0 window, later g_gamma with ghosts
+  it.lv = it.xv + it.VL;  // lambda with ghosts (for g_S)
+  it.rv = rA;
+  it.uv = uA;
+  it.wv = wA;
+  it.wsum = it.lv + it.VL;
+  it.tot = it.wsum + 6 * R::NWAVES;
+  it.ctl = reinterpret_cast<int *>(it.tot + 8);
+  const int total = (int)(it.tot + 8 - lds);
+
+  const int m = it.m, row = it.row, c = it.c, kg = it.kg;
+  const int own_pos = (m + 1) * nP + row;
+  constexpr int VB = 4 * n;  // second edge block
+
+  if (t == 0) {
+    const uint32_t ea = (uint32_t)(xload(it.rec(0, it.rank)) >> 32);
+    const uint32_t eb = (uint32_t)(xload(it.rec(1, it.rank)) >> 32);
+    uint32_t base = (ea > eb ? ea : eb) + 1u;
+    if (base > 0xFFF00000u) base = 1u;
+    it.ctl[0] = 0;
+    it.ctl[1] = (int)base;
+  }
+  for (int i = opaque_tid(); i < total; i += NT) lds[i] = (T)0;
+
+  T mM[CW], mS[CW];
+  if constexpr (BC) {
+    load_row<T, NS>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
+    load_row<T, NS>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
+  } else {
+    load_chunk<T, NS, CW>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+  }
+  __syncthreads();
+  it.base = (uint32_t)it.ctl[1];
+  it.seq = 0;
+  {
+    const T *lam0 = static_cast<const T *>(d.lambda);
+    for (int i = opaque_tid(); i < (K + 2) * n; i += NT) {
+      const int jj = i / n, e = i - jj * n;
+      const int k = it.a - 1 + jj;
+      if (k >= 0 && k < it.N) it.xv[(jj + 1) * nP + e] = gptr(lam0)[(size_t)k * n + e];
+    }
+  }
+  __syncthreads();
+
+  RegOut fo{0, GBDPCG_MAX_ITER, 0.0, 0.0}, ao{0, GBDPCG_MAX_ITER, 0.0, 0.0};
+  bool timed_out = false;
+  T xA = 0, xB = 0, bA = 0, bB = 0;
+  if (it.owner) {
+    xA = it.xv[own_pos];
+    bA = gptr(static_cast<const T *>(d.gamma))[(size_t)kg * n + row];
+    bB = gptr(static_cast<const T *>(d.glambda))[(size_t)kg * n + row];
+  }
+  // r = rhs - S x (B starts from x = 0: r = g_lambda)
+  T rA_i = bA - row_mv(it, mS, it.xv, it.act_w);
+  T rB_i = bB;
+  if (it.owner) {
+    rA[own_pos] = rA_i;
+    rB[own_pos] = rB_i;
+  }
+  it.publish_edge(rA_i, 2);
+  it.publish_edge(rB_i, 2, VB);
+  T tt[6];
+  it.reduce_publish6(it.owner ? bA * bA : (T)0, (T)0, (T)0, it.owner ? bB * bB : (T)0, (T)0, (T)0, true);
+  bool runA = true, runB = true;
+  double thrA = 0.0, thrB = 0.0;
+  T rgA = 0, rgB = 0;
+  T pA = 0, sA = 0, sgA = 0, etaA = 0, alphaA = 0;
+  T pB = 0, sB = 0, sgB = 0, etaB = 0, alphaB = 0;
+  T uA_i = 0, wA_i = 0, uB_i = 0, wB_i = 0;
+  uint32_t iterA = 0, iterB = 0;
+  const uint32_t max_iter = d.max_iter;
+  if (!it.gather_pair(rA, rB, 2, tt)) {
+    timed_out = true;
+  } else {
+    it.xcd = __builtin_amdgcn_readfirstlane(
+                 (int)(it.xcd_ok && it.G > 1 && (double)tt[1] * (double)tt[1] == (double)it.G * (double)tt[2])) != 0;
+    thrA = uniform(it.rel ? it.tol * it.tol * (double)tt[0] : it.tol);
+    thrB = uniform(it.rel ? it.tol * it.tol * (double)tt[3] : it.tol);
+    rgA = it.ghost ? rA[it.gpos] : (T)0;
+    rgB = it.ghost ? rB[it.gpos] : (T)0;
+    // first u = Pinv r, w = S u of both sides
+    uA_i = row_mv(it, mM, rA, it.act_u);
+    uB_i = row_mv(it, mM, rB, it.act_u);
+    if (it.act_u && c == 0) {
+      uA[(m + 1) * nP + row] = uA_i;
+      uB[(m + 1) * nP + row] = uB_i;
+    }
+    __syncthreads();
+    wA_i = row_mv(it, mS, uA, it.act_w);
+    wB_i = row_mv(it, mS, uB, it.act_w);
+    it.publish_edge(wA_i, 2);
+    it.publish_edge(wB_i, 2, VB);
+    it.reduce_publish6(it.owner ? rA_i * uA_i : (T)0, it.owner ? wA_i * uA_i : (T)0, it.owner ? rA_i * rA_i : (T)0,
+                       it.owner ? rB_i * uB_i : (T)0, it.owner ? wB_i * uB_i : (T)0, it.owner ? rB_i * rB_i : (T)0);
+    if (!it.gather_pair(wA, wB, 2, tt)) timed_out = true;
+  }
+  // initial checks of each side (as in reg_item)
+  // last r'z and r'r of each side (uniform, kept as T; widened at the end)
+  T lastA[2] = {0, 0}, lastB[2] = {0, 0};
+  auto first = [&](RegOut &out, bool &run, T eta, T delta, T rr, T w_i, T u_i, T *wv, T &p, T &s_, T &sg, T &et,
+                   T &alpha, T (&last)[2]) {
+    p = u_i;
+    s_ = w_i;
+    sg = it.ghost ? wv[it.gpos] : (T)0;
+    et = eta;
+    alpha = uniform(delta != (T)0 ? eta / delta : (T)0);
+    last[0] = eta;
+    last[1] = rr;
+    bool stop = false;
+    if (!finite_d((double)eta) || !finite_d((double)rr)) {
+      out.status = GBDPCG_NONFINITE;
+      stop = true;
+    } else if (eta == (T)0) {
+      out.status = rr == (T)0 ? GBDPCG_CONVERGED : GBDPCG_BREAKDOWN;
+      stop = true;
+    } else if (delta == (T)0 || !finite_d((double)alpha)) {
+      out.status = GBDPCG_BREAKDOWN;
+      stop = true;
+    }
+    if (stop) run = false;
+    if (max_iter == 0) run = false;
+  };
+  if (!timed_out) {
+    first(fo, runA, tt[0], tt[1], tt[2], wA_i, uA_i, wA, pA, sA, sgA, etaA, alphaA, lastA);
+    first(ao, runB, tt[3], tt[4], tt[5], wB_i, uB_i, wB, pB, sB, sgB, etaB, alphaB, lastB);
+  } else {
+    runA = runB = false;
+  }
+  // The scalar step of one side after an exchange, as in reg_item: candidates
+  // first (both sides in straight-line code, so their dependency chains
+  // overlap), then committed for a running side only.
+  struct Cand {
+    bool conv, nonfin, brk;
+    T alpha, p, s, sg;
+  };
+  auto cand = [&](double thr, T eta_new, T delta_new, T rr, T w_i, T u_i, T w_g, T p, T s_, T sg, T eta,
+                  T alpha) -> Cand {
+    Cand c;
+    c.conv = it.rel ? ((double)rr <= thr) : (__builtin_fabs((double)eta_new) < thr);
+    c.nonfin = !finite_t(eta_new);
+    const T beta = qdiv(eta_new, eta);
+    const T den = delta_new - beta * qdiv(eta_new, alpha);
+    c.brk = den == (T)0 || !finite_t(den);
+    c.alpha = uniform(qdiv(eta_new, den));
+    c.p = fmaT(beta, p, u_i);
+    c.s = fmaT(beta, s_, w_i);
+    c.sg = fmaT(beta, sg, w_g);
+    return c;
+  };
+  // returns whether the side keeps running
+  auto commit = [&](const Cand &c, RegOut &out, uint32_t &iter, T eta_new, T rr, T &p, T &s_, T &sg, T &eta,
+                    T &alpha, T (&last)[2]) -> bool {
+    ++iter;
+    last[0] = eta_new;
+    last[1] = rr;
+    if (c.conv) {
+      out.status = GBDPCG_CONVERGED;
+      return false;
+    }
+    if (c.nonfin) {
+      out.status = GBDPCG_NONFINITE;
+      return false;
+    }
+    if (c.brk) {
+      out.status = GBDPCG_BREAKDOWN;
+      return false;
+    }
+    alpha = c.alpha;
+    eta = eta_new;
+    p = c.p;
+    s_ = c.s;
+    if (it.ghost) sg = c.sg;
+    return iter < max_iter;
+  };
+  // Both sides' vector work runs every iteration (a stopped side's results are
+  // discarded and its state is frozen), so that the compiler can interleave the
+  // two sides' matvecs, reductions and scalar chains.
+  while (runA || runB) {
+    if (it.owner) {
+      const T xa = fmaT(alphaA, pA, xA), ra = fmaT(-alphaA, sA, rA_i);
+      const T xb = fmaT(alphaB, pB, xB), rb = fmaT(-alphaB, sB, rB_i);
+      if (runA) {
+        xA = xa;
+        rA_i = ra;
+      }
+      if (runB) {
+        xB = xb;
+        rB_i = rb;
+      }
+      rA[own_pos] = rA_i;
+      rB[own_pos] = rB_i;
+    }
+    if (it.ghost) {
+      const T ga = fmaT(-alphaA, sgA, rgA), gb = fmaT(-alphaB, sgB, rgB);
+      if (runA) rgA = ga;
+      if (runB) rgB = gb;
+      rA[it.gpos] = rgA;
+      rB[it.gpos] = rgB;
+    }
+    __syncthreads();
+    uA_i = row_mv(it, mM, rA, it.act_u);
+    uB_i = row_mv(it, mM, rB, it.act_u);
+    if (it.act_u && c == 0) {
+      uA[(m + 1) * nP + row] = uA_i;
+      uB[(m + 1) * nP + row] = uB_i;
+    }
+    __syncthreads();
+    wA_i = row_mv(it, mS, uA, it.act_w);
+    wB_i = row_mv(it, mS, uB, it.act_w);
+    // both edge blocks every exchange (a stopped side's are stale and unused):
+    // the neighbours' gather waits for every slot of the current epoch
+    it.publish_edge(wA_i, 2);
+    it.publish_edge(wB_i, 2, VB);
+    it.reduce_publish6(it.owner ? rA_i * uA_i : (T)0, it.owner ? wA_i * uA_i : (T)0, it.owner ? rA_i * rA_i : (T)0,
+                       it.owner ? rB_i * uB_i : (T)0, it.owner ? wB_i * uB_i : (T)0, it.owner ? rB_i * rB_i : (T)0);
+    if (!it.gather_pair(wA, wB, 2, tt)) {
+      timed_out = true;
+      break;
+    }
+    const T wgA = it.ghost ? wA[it.gpos] : (T)0;
+    const T wgB = it.ghost ? wB[it.gpos] : (T)0;
+    const Cand cA = cand(thrA, tt[0], tt[1], tt[2], wA_i, uA_i, wgA, pA, sA, sgA, etaA, alphaA);
+    const Cand cB = cand(thrB, tt[3], tt[4], tt[5], wB_i, uB_i, wgB, pB, sB, sgB, etaB, alphaB);
+    if (runA) runA = commit(cA, fo, iterA, tt[0], tt[2], pA, sA, sgA, etaA, alphaA, lastA);
+    if (runB) runB = commit(cB, ao, iterB, tt[3], tt[5], pB, sB, sgB, etaB, alphaB, lastB);
+  }
+  fo.iters = (int)iterA;
+  ao.iters = (int)iterB;
+  fo.eta = (double)lastA[0];
+  fo.rr = (double)lastA[1];
+  ao.eta = (double)lastB[0];
+  ao.rr = (double)lastB[1];
+  if (timed_out) {
+    fo.status = GBDPCG_TIMED_OUT;
+    ao.status = GBDPCG_TIMED_OUT;
+  } else {
+    // lambda and g_gamma to HBM and, with one ghost knot each side, to LDS
+    if (it.owner) {
+      gptr(static_cast<T *>(d.lambda))[(size_t)kg * n + row] = xA;
+      gptr(static_cast<T *>(d.ggamma))[(size_t)kg * n + row] = xB;
+      it.lv[own_pos] = xA;
+      it.xv[own_pos] = xB;
+    }
+    it.publish_edge(xB, 1);
+    it.publish_edge(xA, 1, VB);
+    it.reduce_publish6((T)0, (T)0, (T)0, (T)0, (T)0, (T)0);
+    if (!it.gather_pair(it.xv, it.lv, 1, tt)) {
+      timed_out = true;
+      fo.status = GBDPCG_TIMED_OUT;
+      ao.status = GBDPCG_TIMED_OUT;
+    } else {
+      // g_S block rows of the own knots: -1/2 (gg_k lam_j^T + lam_k gg_j^T), j in {k-1,k,k+1}
+      const int bs = 3 * n * n;
+      T *gS = static_cast<T *>(d.gS);
+      for (int e = opaque_tid(); e < K * bs; e += NT) {
+        const int kl = e / bs;
+        const int wq = e - kl * bs;
+        const int col = wq / n;
+        const int rr_ = wq - col * n;
+        const int seg = col / n;
+        const int cc = col - seg * n;
+        const int k = it.a + kl;
+        const int j = k - 1 + seg;
+        T val = 0;
+        if (j >= 0 && j < it.N) {
+          const int lk = (kl + 2) * nP, lj = (kl + 1 + seg) * nP;
+          val = (T)-0.5 * (it.xv[lk + rr_] * it.lv[lj + cc] + it.lv[lk + rr_] * it.xv[lj + cc]);
+        }
+        gptr(gS)[(size_t)it.a * bs + e] = val;
+      }
+    }
+  }
+  __syncthreads();
+  if (rank == 0 && t == 0 && d.result != nullptr) {
+    gbdpcg_result res;
+    res.iters = fo.iters;
+    res.status = fo.status;
+    res.adj_iters = ao.iters;
+    res.adj_status = ao.status;
+    res.eta = fo.eta;
+    res.rr = fo.rr;
+    res.adj_eta = ao.eta;
+    res.adj_rr = ao.rr;
+    *d.result = res;
+  }
+  if (timed_out && t == 0 && d.err != nullptr)
+    __hip_atomic_fetch_or(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace
 
 // NS != 0: the kernel is specialised for stateSize NS.  NS == 0: runtime
 // dispatch over the specialised sizes (batches of uneven stateSize).
-template <typename T, int NS, int NT, bool FUSED>
+template <typename T, int NS, int NT, bool FUSED, bool BC>
 __device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *lds) {
   if constexpr (NS != 0) {
-    reg_item<T, NS, NT, FUSED>(d, rank, lds);
+    reg_item<T, NS, NT, FUSED, BC>(d, rank, lds);
+  } else if constexpr (BC) {
+    switch (d.n) {
+      case 12: reg_item<T, 12, NT, FUSED, true>(d, rank, lds); break;
+      case 14: reg_item<T, 14, NT, FUSED, true>(d, rank, lds); break;
+      default: break;  // the planner only admits the sizes above
+    }
   } else {
     switch (d.n) {
       case 2: reg_item<T, 2, NT, FUSED>(d, rank, lds); break;
@@ -723,17 +1280,16 @@ __device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *ld
 
 // Single system (systems == nullptr): workgroup b is rank b/stride when
 // b % stride == 0.  Batched: items[round * gridDim.x + b] as in pcg_kernel.
-template <typename T, int NS, int NT, bool FUSED>
-__global__ void __launch_bounds__(NT) pcg_reg_kernel(KSys one, const KSys *__restrict__ systems,
-                                                     const uint32_t *__restrict__ items, uint32_t rounds,
-                                                     uint32_t stride) {
+template <typename T, int NS, int NT, bool FUSED, bool BC>
+__device__ __forceinline__ void reg_entry(const KSys &one, const KSys *__restrict__ systems,
+                                          const uint32_t *__restrict__ items, uint32_t rounds, uint32_t stride) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   T *lds = reinterpret_cast<T *>(lds_raw);
   if (systems == nullptr) {
     if (blockIdx.x % stride) return;
     const uint32_t rank = blockIdx.x / stride;
     if (rank >= one.G) return;
-    reg_dispatch<T, NS, NT, FUSED>(one, rank, lds);
+    reg_dispatch<T, NS, NT, FUSED, BC>(one, rank, lds);
     return;
   }
   for (uint32_t rd = 0; rd < rounds; ++rd) {
@@ -741,7 +1297,89 @@ __global__ void __launch_bounds__(NT) pcg_reg_kernel(KSys one, const KSys *__res
     if (item == kNoItem) continue;
     __syncthreads();
     const KSys d = systems[item >> 12];
-    reg_dispatch<T, NS, NT, FUSED>(d, item & 0xFFFu, lds);
+    reg_dispatch<T, NS, NT, FUSED, BC>(d, item & 0xFFFu, lds);
+  }
+}
+
+// Register-resident rows, reg_lpr(n) lanes per row, column chunks.
+template <typename T, int NS, int NT, bool FUSED>
+__global__ void __launch_bounds__(NT) pcg_reg_kernel(KSys one, const KSys *__restrict__ systems,
+                                                     const uint32_t *__restrict__ items, uint32_t rounds,
+                                                     uint32_t stride) {
+  reg_entry<T, NS, NT, FUSED, false>(one, systems, items, rounds, stride);
+}
+
+// Register-resident rows, row-broadcast layout (n in {12, 14}): one lane per row,
+// 16 lanes per knot, the vector window broadcast within each 16-lane row by DPP.
+template <typename T, int NS, int NT, bool FUSED>
+__global__ void __launch_bounds__(NT) pcg_rowcast_kernel(KSys one, const KSys *__restrict__ systems,
+                                                         const uint32_t *__restrict__ items, uint32_t rounds,
+                                                         uint32_t stride) {
+  reg_entry<T, NS, NT, FUSED, true>(one, systems, items, rounds, stride);
+}
+
+// Paired forward + adjoint items (fused recurrence, both flags set) go to
+// reg_item_pair, every other item to reg_item.
+template <typename T, int NS, int NT, bool BC>
+__device__ __forceinline__ void pair_item(const KSys &d, uint32_t rank, T *lds) {
+  if ((d.flags & (KF_FWD | KF_ADJ)) == (KF_FWD | KF_ADJ) && (d.flags & KF_FUSED))
+    reg_item_pair<T, NS, NT, BC>(d, rank, lds);
+  else
+    reg_item<T, NS, NT, true, BC>(d, rank, lds);
+}
+
+template <typename T, int NS, int NT, bool BC>
+__device__ __forceinline__ void pair_dispatch(const KSys &d, uint32_t rank, T *lds) {
+  if constexpr (NS != 0) {
+    pair_item<T, NS, NT, BC>(d, rank, lds);
+  } else if constexpr (BC) {
+    switch (d.n) {
+      case 12: pair_item<T, 12, NT, true>(d, rank, lds); break;
+      case 14: pair_item<T, 14, NT, true>(d, rank, lds); break;
+      default: break;
+    }
+  } else {
+    switch (d.n) {  // paired plans hold rows of n <= 14 only
+      case 2: pair_item<T, 2, NT, false>(d, rank, lds); break;
+      case 6: pair_item<T, 6, NT, false>(d, rank, lds); break;
+      case 12: pair_item<T, 12, NT, false>(d, rank, lds); break;
+      case 14: pair_item<T, 14, NT, false>(d, rank, lds); break;
+      default: break;
+    }
+  }
+}
+
+// Waves per SIMD the paired kernel is compiled for: FP32 column-chunk rows of
+// n <= 14 at 512 threads are held to 128 VGPRs, so that two workgroups share a
+// CU (a batch of 64 systems of N = 128 then runs in one round); the compiler
+// spills what does not fit.
+template <typename T, int NS, int NT, bool BC>
+constexpr int pair_waves_per_eu() {
+  return (sizeof(T) == 4 && NT == 512 && !BC && NS != 0 && NS <= 14) ? 4 : 1;
+}
+
+// Forward + adjoint solves iterated together (see reg_item_pair); same launch
+// interface as pcg_reg_kernel.
+template <typename T, int NS, int NT, bool BC>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(pair_waves_per_eu<T, NS, NT, BC>())))
+pcg_pair_kernel(KSys one, const KSys *__restrict__ systems,
+                                                      const uint32_t *__restrict__ items, uint32_t rounds,
+                                                      uint32_t stride) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  T *lds = reinterpret_cast<T *>(lds_raw);
+  if (systems == nullptr) {
+    if (blockIdx.x % stride) return;
+    const uint32_t rank = blockIdx.x / stride;
+    if (rank >= one.G) return;
+    pair_dispatch<T, NS, NT, BC>(one, rank, lds);
+    return;
+  }
+  for (uint32_t rd = 0; rd < rounds; ++rd) {
+    const uint32_t item = items[(size_t)rd * gridDim.x + blockIdx.x];
+    if (item == kNoItem) continue;
+    __syncthreads();
+    const KSys d = systems[item >> 12];
+    pair_dispatch<T, NS, NT, BC>(d, item & 0xFFFu, lds);
   }
 }
 

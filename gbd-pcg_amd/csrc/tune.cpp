@@ -210,6 +210,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
     gbdpcg_config c = *cfg;
     c.knots_per_wg = hit.a;
     c.block[0] = hit.b;
+    c.kernel = hit.c;
     int rc = gbdpcg_batch_create(out, dtype, systems, count, &c, 0);
     if (rc == GBDPCG_OK) {
       gbdpcg_batch_info info{};
@@ -218,14 +219,14 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
       rep.knots_per_wg = hit.a;
       rep.threads = (uint32_t)batch_threads(*out);
       rep.wgs = info.grid;
-      rep.kernel = hit.c;
+      rep.kernel = (uint32_t)batch_kernel(*out);
       if (report) *report = rep;
       return GBDPCG_OK;
     }
     // the cached choice no longer plans (other device limits): tune again
   }
 
-  // candidate knobs: knots-per-workgroup caps x threads per workgroup
+  // candidate knobs: kernel x threads per workgroup x knots-per-workgroup caps
   bool same_n = true;
   uint32_t maxN = 0;
   for (uint32_t i = 0; i < count; ++i) {
@@ -242,30 +243,38 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   if (rc < 0) return rc;
   HIPCHK(hipStreamSynchronize(st));
 
+  // the untuned plan's kernel first (its cap-0 plan is the default), then the
+  // row-broadcast layout when the kernel is automatic
+  std::vector<uint32_t> kernels = {cfg->kernel};
+  if (cfg->kernel == GBDPCG_KERNEL_AUTO) kernels.push_back(GBDPCG_KERNEL_ROWCAST);
+
   gbdpcg_batch *best = nullptr;
   double best_us = 1e300, default_us = -1.0;
-  uint32_t best_cap = 0, best_nt = 512;
+  uint32_t best_cap = 0, best_nt = 512, best_kernel = cfg->kernel;
   std::vector<std::vector<uint32_t>> seen;  // per-system workgroup vectors already timed
-  std::vector<int> seen_nt;                 // ... and the threads per workgroup they ran with
+  std::vector<int> seen_nt, seen_k;         // ... and the threads per workgroup and kernel they ran with
+  for (uint32_t kid : kernels)
   for (uint32_t nt : nts) {
     for (uint32_t cap : caps) {
       if (cap > maxN) continue;
       gbdpcg_config c = *cfg;
       c.knots_per_wg = cap;
       c.block[0] = nt;
+      c.kernel = kid;
       gbdpcg_batch *p = nullptr;
       if (gbdpcg_batch_create(&p, dtype, systems, count, &c, 0) != GBDPCG_OK) continue;  // does not plan
       std::vector<uint32_t> w(count);
       gbdpcg_batch_wgs(p, w.data());
       bool dup = false;
       for (size_t j = 0; j < seen.size(); ++j)
-        if (seen[j] == w && seen_nt[j] == batch_threads(p)) dup = true;
+        if (seen[j] == w && seen_nt[j] == batch_threads(p) && seen_k[j] == batch_kernel(p)) dup = true;
       if (dup) {
         gbdpcg_batch_destroy(p);
         continue;
       }
       seen.push_back(w);
       seen_nt.push_back(batch_threads(p));
+      seen_k.push_back(batch_kernel(p));
       double us = 0.0;
       rc = time_plan(p, count, snap, trials, st, &us);
       if (rc < 0) {
@@ -276,13 +285,14 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
         return rc;
       }
       ++rep.candidates;
-      if (cap == 0 && nt == def_nt) default_us = us;
+      if (cap == 0 && nt == def_nt && kid == cfg->kernel) default_us = us;
       if (us < best_us) {
         if (best) gbdpcg_batch_destroy(best);
         best = p;
         best_us = us;
         best_cap = cap;
         best_nt = nt;
+        best_kernel = kid;
       } else {
         gbdpcg_batch_destroy(p);
       }
@@ -303,10 +313,10 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   rep.knots_per_wg = best_cap;
   rep.threads = (uint32_t)batch_threads(best);
   rep.wgs = info.grid;
-  rep.kernel = batch_uses_registers(best) ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+  rep.kernel = (uint32_t)batch_kernel(best);
   rep.best_us = best_us;
   rep.default_us = default_us;
-  tune_store(key, TuneChoice{best_cap, best_nt, rep.kernel, 0});
+  tune_store(key, TuneChoice{best_cap, best_nt, best_kernel, 0});
   if (report) *report = rep;
   *out = best;
   return GBDPCG_OK;

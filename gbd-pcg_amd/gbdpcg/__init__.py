@@ -29,7 +29,7 @@ F32, F64 = 0, 1
 EXIT_ABS_ETA, EXIT_REL_RESID = 0, 1
 PRECOND_JACOBI, PRECOND_SS = 1, 2
 RECUR_REFERENCE, RECUR_FUSED = 0, 1
-KERNEL_AUTO, KERNEL_LDS, KERNEL_REGISTER = 0, 1, 2
+KERNEL_AUTO, KERNEL_LDS, KERNEL_REGISTER, KERNEL_ROWCAST = 0, 1, 2, 3
 SYS_FORWARD, SYS_ADJOINT = 1, 2
 
 _ERRNAMES = {ERR_INVALID: "INVALID", ERR_TOO_LARGE: "TOO_LARGE", ERR_DEVICE: "DEVICE", ERR_TIMEOUT: "TIMEOUT",

@@ -96,8 +96,11 @@ typedef struct gbdpcg_config {
 /* kernel selection (gbdpcg_config.kernel) */
 #define GBDPCG_KERNEL_AUTO 0
 #define GBDPCG_KERNEL_LDS 1      /* LDS-resident block rows, both recurrences, any n  */
-#define GBDPCG_KERNEL_REGISTER 2 /* register-resident rows, FUSED recurrence only,
+#define GBDPCG_KERNEL_REGISTER 2 /* register-resident rows (column chunks),
                                     n in {2, 6, 12, 14, 28}                           */
+#define GBDPCG_KERNEL_ROWCAST 3  /* register-resident rows, one lane per row and 16
+                                    lanes per knot, the vector window broadcast by
+                                    DPP within each 16-lane row; n in {12, 14}        */
 
 typedef struct gbdpcg_result {
   int32_t iters;      /* forward iterations (reference counting, pcg.cuh:195,212)   */
@@ -274,7 +277,7 @@ typedef struct gbdpcg_tune_report {
   uint32_t knots_per_wg;   /* chosen cap on knots per workgroup                        */
   uint32_t threads;        /* chosen threads per workgroup                             */
   uint32_t wgs;            /* single solve: chosen workgroups; batch: launch grid      */
-  uint32_t kernel;         /* GBDPCG_KERNEL_LDS or GBDPCG_KERNEL_REGISTER              */
+  uint32_t kernel;         /* GBDPCG_KERNEL_LDS, _REGISTER or _ROWCAST                 */
   double best_us;          /* mean time of one solve / one batch run, chosen layout    */
   double default_us;       /* same for the untuned default layout                      */
 } gbdpcg_tune_report;

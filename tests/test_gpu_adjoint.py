@@ -83,6 +83,47 @@ def test_adjoint_against_oracle(gpu, dtype, recurrence, N):
     assert rel(gg, gg_x) <= 1e-4 and rel(gS, gS_x) <= 1e-4
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+def test_sides_stop_independently(gpu, dtype, kernel):
+    """Forward + adjoint, fused recurrence (the register kernels run both solves in
+    one loop): each side stops on its own test, keeps its own status and count,
+    and does not disturb the other."""
+    import gbdpcg
+    n, N = 14, 64
+    S, P, g, glam = system(n, N, 4321, dtype)
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    bar = 1e-4 if dtype == np.float32 else 1e-10
+    kw = dict(tol=tol, max_iter=173, exit_mode=1, recurrence=1, kernel=kernel)
+    zero = np.zeros(n * N)
+
+    def oracle_iters(rhs):
+        return O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), np.zeros(n * N, dtype), n, N, tol, 173, 1,
+                     dtype, fused=True)[1]
+
+    # g_lambda = 0: the adjoint is converged at iteration 0 while the forward solve runs
+    lam_a, gg, gS, res = run_adjoint(S, P, g, zero, n, N, dtype, **kw)
+    assert (res.adj_iters, res.adj_status) == (0, gbdpcg.CONVERGED) and not gg.any() and not gS.any()
+    assert res.status == gbdpcg.CONVERGED and abs(res.iters - oracle_iters(g)) <= 1
+    assert rel(lam_a, O.pcg(S, P, g, zero, n, N, 0.0, res.iters)[0]) <= bar
+    # gamma = 0: the forward solve is converged at iteration 0 while the adjoint runs
+    lam, gg, gS, res = run_adjoint(S, P, zero, glam, n, N, dtype, **kw)
+    assert (res.iters, res.status) == (0, gbdpcg.CONVERGED) and not lam.any() and not gS.any()
+    assert res.adj_status == gbdpcg.CONVERGED and abs(res.adj_iters - oracle_iters(glam)) <= 1
+    assert rel(gg, O.pcg(S, P, glam, zero, n, N, 0.0, res.adj_iters)[0]) <= bar
+    # a NaN in g_lambda: the adjoint ends NONFINITE, the forward solve is unchanged
+    bad = glam.copy()
+    bad[7] = np.nan
+    lam_c, gg, gS, res = run_adjoint(S, P, g, bad, n, N, dtype, **kw)
+    assert res.adj_status == gbdpcg.NONFINITE and res.status == gbdpcg.CONVERGED
+    assert np.array_equal(lam_c, lam_a)
+    # max_iter = 5: both sides stop there, each at the oracle's 5-iteration iterate
+    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, **dict(kw, max_iter=5))
+    assert (res.iters, res.status, res.adj_iters, res.adj_status) == (5, gbdpcg.MAX_ITER, 5, gbdpcg.MAX_ITER)
+    assert rel(lam, O.pcg(S, P, g, zero, n, N, 0.0, 5)[0]) <= bar
+    assert rel(gg, O.pcg(S, P, glam, zero, n, N, 0.0, 5)[0]) <= bar
+
+
 @pytest.mark.parametrize("recurrence", [0, 1])
 def test_adjoint_only_entry(gpu, recurrence):
     """forward = 0: lambda is an input; same g_gamma/g_S as the fused launch."""

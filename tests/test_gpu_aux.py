@@ -190,11 +190,12 @@ def test_async_result_record(gpu):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [2, 6, 12, 14, 28])
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
 @pytest.mark.parametrize("recurrence", [0, 1])
 def test_forced_kernels(gpu, dtype, n, kernel, recurrence):
-    """Both recurrences on the LDS-resident and on the register-resident kernel,
-    each against the oracle (iterations within 1, lambda to the precision bar)."""
+    """Both recurrences on the LDS-resident kernel and on both register-resident
+    layouts (column chunks; row broadcast), each against the oracle (iterations
+    within 1, lambda to the precision bar)."""
     import gbdpcg
     N = 64
     S, g = lqr(n, N, 500 + n)
@@ -206,10 +207,13 @@ def test_forced_kernels(gpu, dtype, n, kernel, recurrence):
         r = gbdpcg.solve(dev(Sd, dtype), dev(gd, dtype), lam, n, N, dev(Pd, dtype), tol=tol, max_iter=400,
                          exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence, kernel=kernel)
     except gbdpcg.GbdpcgError as e:
-        # the register kernel holds FP64 rows of n > 14 only at 256 threads; it is
-        # allowed to decline, but only with UNSUPPORTED
-        assert kernel == 2 and e.code == gbdpcg.ERR_UNSUPPORTED
+        # the register kernel holds FP64 rows of n > 14 only at 256 threads, and the
+        # row-broadcast layout takes n in {12, 14} only; they may decline, but only
+        # with UNSUPPORTED
+        assert e.code == gbdpcg.ERR_UNSUPPORTED
+        assert (kernel == 2 and dtype == np.float64 and n > 14) or (kernel == 3 and n not in (12, 14))
         return
+    assert kernel != 3 or n in (12, 14)
     assert r.status == gbdpcg.CONVERGED
     _, it_same, *_ = O.pcg(Sd, Pd, gd, np.zeros(n * N, dtype), n, N, tol, 400, O.EXIT_REL, dtype,
                            fused=recurrence == 1)
@@ -229,7 +233,7 @@ def test_register_kernel_declines_unsupported(gpu):
     assert e.value.code == gbdpcg.ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("recurrence,kernel", [(0, 0), (1, 1), (1, 2)])
+@pytest.mark.parametrize("recurrence,kernel", [(0, 0), (1, 1), (1, 2), (0, 3), (1, 3)])
 def test_nonfinite_input(gpu, recurrence, kernel):
     """A NaN in gamma ends the solve with status NONFINITE (no hang, no exception)."""
     import gbdpcg

@@ -81,6 +81,21 @@ def test_forward_adjoint_batch(gpu, recurrence):
     check(systems, host, b.results(), np.float32, recurrence, adjoint=True)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [12, 14])
+@pytest.mark.parametrize("kernel", [2, 3])
+def test_register_layouts_forward_adjoint(gpu, dtype, n, kernel):
+    """Batches of one stateSize, forward + adjoint, forced onto each register
+    layout (column chunks; row broadcast), uneven N and workgroups of 1..30 knots."""
+    import gbdpcg
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    specs = [(n, N, tol) for N in (4, 16, 33, 64, 97, 128, 200)]
+    systems, host = make_batch(specs, dtype, adjoint=True, seed0=9300 + n)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1, kernel=kernel)
+    b.run()
+    check(systems, host, b.results(), dtype, 1, adjoint=True)
+
+
 def test_repeated_runs_and_small_lds_budget(gpu):
     """Runs continue the exchange epochs; a small LDS budget forces more workgroups
     per system and several rounds per workgroup."""

@@ -32,7 +32,7 @@ def test_tune_single_then_solve(gpu, recurrence, N):
     rep = gbdpcg.tune_single(dS, dg, n, N, dP, **kw)
     assert rep["candidates"] >= 4 and rep["from_cache"] == 0
     assert 0 < rep["best_us"] <= rep["default_us"]
-    assert rep["kernel"] in (gbdpcg.KERNEL_LDS, gbdpcg.KERNEL_REGISTER)
+    assert rep["kernel"] in (gbdpcg.KERNEL_LDS, gbdpcg.KERNEL_REGISTER, gbdpcg.KERNEL_ROWCAST)
     lam = _t(np.zeros(n * N), dtype)
     r = gbdpcg.solve(dS, dg, lam, n, N, dP, **kw)
     assert r.status == gbdpcg.CONVERGED
