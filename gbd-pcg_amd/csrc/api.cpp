@@ -275,9 +275,10 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
   if (rc < 0) return rc;
   Layout spread;  // first layout found whose workgroups span several XCDs
   bool have_spread = false;
-  for (int nt : {256, 512}) {
+  for (int nt : {256, 512, 64}) {
     if (only_nt && nt != only_nt) continue;
-    if (!fused && nt != 256) continue;  // reference-recurrence instances: 256 threads
+    if (nt == 64 && (!bc || !only_nt)) continue;  // one-wave ranks: on request (or tuned)
+    if (!fused && nt == 512) continue;  // reference-recurrence instances: 64 and 256 threads
     // FP64 rows of n = 28 (and whole FP64 rows in the row-broadcast layout) need
     // more VGPRs than two waves per SIMD leave: one wave per SIMD only
     if (dtype == GBDPCG_F64 && (n > 14 || bc) && nt > 256) continue;
@@ -349,7 +350,7 @@ static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, bool bc, int
   int occ = 0;
   if (reg) {
     if (!reg_supported((int)n, t.bc) || ceil_div(N, G) > reg_max_knots(dtype, n, nt, t.bc) || N / G < 2 ||
-        (!fused && nt != 256) || (t.bc && dtype == GBDPCG_F64 && nt != 256))
+        (!fused && nt == 512) || (nt == 64 && (!t.bc || t.pair)) || (t.bc && dtype == GBDPCG_F64 && nt == 512))
       return GBDPCG_ERR_UNSUPPORTED;
     t.lds = reg_lds_bytes(dtype, n, ceil_div(N, G), nt, rv);
     if (t.lds > di.max_lds) return GBDPCG_ERR_UNSUPPORTED;
@@ -372,21 +373,23 @@ static int choose_layout(int dev, int dtype, uint32_t n, uint32_t N, bool fused,
                          Layout *L, bool pair = false) {
   if (cfg->grid[0] == 0 && cfg->knots_per_wg == 0 && cfg->kernel == GBDPCG_KERNEL_AUTO) {
     TuneChoice c;
-    if (tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
+    if (cfg->threads == 0 && tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
         layout_from(dtype, n, N, c.a != 0, c.a == 2, (int)c.b, c.c, c.d, L, fused, pair) == GBDPCG_OK)
       return GBDPCG_OK;
   }
   // automatic: the row-broadcast layout first for FP64 (measured faster there),
-  // the column-chunk register layout first for FP32
+  // the column-chunk register layout first for FP32.  cfg->threads asks the
+  // register kernels for that many threads per workgroup.
+  const int req_nt = (int)cfg->threads;
   int rc = GBDPCG_ERR_UNSUPPORTED;
   if ((cfg->kernel == GBDPCG_KERNEL_AUTO && dtype == GBDPCG_F64) || cfg->kernel == GBDPCG_KERNEL_ROWCAST)
-    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, 0, fused, true, pair);
+    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, req_nt, fused, true, pair);
   if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_ROWCAST) {
     set_error("row-broadcast kernel needs n in {12,14}, N >= 4 and a fitting layout");
     return GBDPCG_ERR_UNSUPPORTED;
   }
   if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_REGISTER))
-    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, 0, fused, false, pair);
+    rc = plan_single_reg(dtype, n, N, cfg->grid[0], cfg->knots_per_wg, L, req_nt, fused, false, pair);
   if (rc == GBDPCG_ERR_UNSUPPORTED && cfg->kernel == GBDPCG_KERNEL_REGISTER) {
     set_error("register kernel needs n in {2,6,12,14,28}, N >= 4 and a fitting layout");
     return GBDPCG_ERR_UNSUPPORTED;
@@ -418,6 +421,10 @@ static int validate_cfg(const gbdpcg_config *cfg) {
   }
   if (cfg->kernel > GBDPCG_KERNEL_ROWCAST) {
     set_error("bad kernel %u", cfg->kernel);
+    return GBDPCG_ERR_INVALID;
+  }
+  if (cfg->threads != 0 && cfg->threads != 64 && cfg->threads != 256 && cfg->threads != 512) {
+    set_error("bad threads %u (0, 64, 256 or 512)", cfg->threads);
     return GBDPCG_ERR_INVALID;
   }
   if (cfg->precond != GBDPCG_PRECOND_JACOBI && cfg->precond != GBDPCG_PRECOND_SS) {
@@ -726,8 +733,10 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   for (int kind = 2; kind >= 0; --kind) {
     const uint32_t kid = kind == 2 ? GBDPCG_KERNEL_ROWCAST : kind == 1 ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
     if (cfg->kernel != GBDPCG_KERNEL_AUTO && cfg->kernel != kid) continue;
-    for (int nt : {256, 512}) {
+    if (kind == 0 && cfg->threads) continue;  // a thread request names a register kernel
+    for (int nt : {256, 512, 64}) {
       if (!kind && nt != kThreads) continue;
+      if (nt == 64 && kind != 2) continue;  // one-wave ranks: row-broadcast layout only
       for (uint32_t K : Ks) {
         const uint32_t G = ceil_div(N, K);
         if (G > (uint32_t)kMaxWgsPerSystem || (fused && N / G < 2)) continue;

@@ -166,7 +166,10 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   const RegVariant rv{true, bc, pair};
   // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads only
   const bool only256 = f64_wide || (bc && dtype == GBDPCG_F64);
-  const int nt = (use_reg && same_n && (cfg->block[0] == 256 || only256)) ? 256 : 512;
+  // threads per workgroup: cfg->threads (or the reference-style block[0]) = 256
+  // asks for 256 when every system shares n
+  const bool want256 = cfg->threads == 256 || (cfg->threads == 0 && cfg->block[0] == 256);
+  const int nt = (use_reg && same_n && (want256 || only256)) ? 256 : 512;
   const int kn = same_n ? (int)systems[0].n : 0;
 
   // workgroups per system and the launch's LDS size

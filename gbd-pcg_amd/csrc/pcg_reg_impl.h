@@ -169,11 +169,107 @@ struct RegItem {
     }
   }
 
+  // Source word and LDS destination of edge entry i (payload word i of the
+  // neighbours' edge knots, left neighbour's first) of edge block voff.
+  __device__ __forceinline__ void edge_slot(int i, int nleft, int type, int voff, T *vec, uint64_t *&src,
+                                            uint32_t *&dst) const {
+    if (i < nleft) {  // left neighbour's right edge: knot a-1-e = local 1-e
+      const int vi = i / WPV, qq = i - vi * WPV;
+      const int e = vi / n, rr = vi - e * n;
+      src = rec(type, rank - 1) + (kRecScalars + voff + 2 * n + e * n + rr) * WPV + qq;
+      dst = reinterpret_cast<uint32_t *>(vec + (1 - e) * nP + rr) + qq;
+    } else {  // right neighbour's left edge: knot b+e = local K+2+e
+      const int ii = i - nleft;
+      const int vi = ii / WPV, qq = ii - vi * WPV;
+      const int e = vi / n, rr = vi - e * n;
+      src = rec(type, rank + 1) + (kRecScalars + voff + e * n + rr) * WPV + qq;
+      dst = reinterpret_cast<uint32_t *>(vec + (K + 2 + e) * nP + rr) + qq;
+    }
+  }
+
+  // One-wave workgroups: lane t polls rank t's three totals and edge entries t
+  // and t + 64 in one spin loop (one round trip when everything has landed);
+  // ranks beyond 64 are polled afterwards.  Sums as the multi-wave gather does.
+  __device__ __forceinline__ bool gather_one_wave(T *vec, int edge, uint32_t epoch, int type, int t) {
+    constexpr int NW = 3 * WPV;
+    const int en = edge * n * WPV;
+    const int nleft = rank > 0 ? en : 0;
+    const int ne = nleft + (rank < G - 1 ? en : 0);  // <= 4 * 16 * 2 = 128
+    uint64_t *src0 = nullptr, *src1 = nullptr;
+    uint32_t *dst0 = nullptr, *dst1 = nullptr;
+    if (t < ne) edge_slot(t, nleft, type, 0, vec, src0, dst0);
+    if (t + 64 < ne) edge_slot(t + 64, nleft, type, 0, vec, src1, dst1);
+    bool need_s = t < G, need0 = src0 != nullptr, need1 = src1 != nullptr;
+    uint32_t wv_[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) wv_[i] = 0;
+    uint32_t v0 = 0, v1 = 0;
+    for (uint32_t spin = 0; spin < kSpinLimit && (need_s || need0 || need1); ++spin) {
+      if (need_s) {
+        uint64_t A[NW];
+        uint64_t *p = rec(type, t);
+#pragma unroll
+        for (int i = 0; i < NW; ++i) A[i] = xload(p + i);
+        bool all = true;
+#pragma unroll
+        for (int i = 0; i < NW; ++i) all = all && (uint32_t)(A[i] >> 32) == epoch;
+        if (all) {
+#pragma unroll
+          for (int i = 0; i < NW; ++i) wv_[i] = (uint32_t)A[i];
+          need_s = false;
+        }
+      }
+      if (need0) {
+        const uint64_t w = xload(src0);
+        if ((uint32_t)(w >> 32) == epoch) {
+          v0 = (uint32_t)w;
+          need0 = false;
+        }
+      }
+      if (need1) {
+        const uint64_t w = xload(src1);
+        if ((uint32_t)(w >> 32) == epoch) {
+          v1 = (uint32_t)w;
+          need1 = false;
+        }
+      }
+    }
+    bool ok = !(need_s || need0 || need1);
+    if (src0 && !need0) *dst0 = v0;
+    if (src1 && !need1) *dst1 = v1;
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    if (t < G && !need_s) {
+      acc0 = word_value(wv_, 0);
+      acc1 = word_value(wv_, 1);
+      acc2 = word_value(wv_, 2);
+    }
+    for (int g = t + 64; g < G; g += 64) {
+      uint32_t w2[NW];
+      if (!xwait_words<NW>(rec(type, g), epoch, w2)) {
+        ok = false;
+        break;
+      }
+      acc0 += word_value(w2, 0);
+      acc1 += word_value(w2, 1);
+      acc2 += word_value(w2, 2);
+    }
+    acc0 = wave_sum(acc0);
+    acc1 = wave_sum(acc1);
+    acc2 = wave_sum(acc2);
+    if (t == 0) {
+      tot[0] = acc0;
+      tot[1] = acc1;
+      tot[2] = acc2;
+    }
+    return ok;
+  }
+
   // Polls the current exchange.  Wave 0 gathers every rank's wave totals and
   // reduces them (waves in order, then ranks by a fixed DPP tree) into tot; the
   // other waves copy the neighbours' `edge` edge knots into the ghost knots of
-  // `vec`.  Ends with a barrier after which t0..t2 hold the totals in every
-  // thread; returns false (uniformly) if a wait hit its bound.
+  // `vec` (one-wave workgroups: gather_one_wave).  Ends with a barrier after
+  // which t0..t2 hold the totals in every thread; returns false (uniformly) if
+  // a wait hit its bound.
   __device__ __forceinline__ bool gather(T *vec, int edge, T &t0, T &t1, T &t2) {
     if (G == 1) {
       __syncthreads();
@@ -187,7 +283,9 @@ struct RegItem {
     const int type = (int)(seq & 1u);
     bool ok = true;
     const int t = opaque_tid();
-    if (t < 64) {
+    if constexpr (NT == 64) {
+      ok = gather_one_wave(vec, edge, epoch, type, t);
+    } else if (t < 64) {
       constexpr int NW = 3 * WPV;
       T acc0 = 0, acc1 = 0, acc2 = 0;
       for (int g = t; g < G; g += 64) {
@@ -215,18 +313,7 @@ struct RegItem {
       for (int i = t - 64; i < nleft + nright; i += NT - 64) {
         uint64_t *src;
         uint32_t *dst;
-        if (i < nleft) {  // left neighbour's right edge: knot a-1-e = local 1-e
-          const int vi = i / WPV, qq = i - vi * WPV;
-          const int e = vi / n, rr = vi - e * n;
-          src = rec(type, rank - 1) + (kRecScalars + 2 * n + e * n + rr) * WPV + qq;
-          dst = reinterpret_cast<uint32_t *>(vec + (1 - e) * nP + rr) + qq;
-        } else {  // right neighbour's left edge: knot b+e = local K+2+e
-          const int ii = i - nleft;
-          const int vi = ii / WPV, qq = ii - vi * WPV;
-          const int e = vi / n, rr = vi - e * n;
-          src = rec(type, rank + 1) + (kRecScalars + e * n + rr) * WPV + qq;
-          dst = reinterpret_cast<uint32_t *>(vec + (K + 2 + e) * nP + rr) + qq;
-        }
+        edge_slot(i, nleft, type, 0, vec, src, dst);
         uint32_t val = 0;
         if (xcd ? xwait(src, epoch, val) : xwait_lazy(src, epoch, val)) {
           *dst = val;
@@ -322,22 +409,9 @@ struct RegItem {
       for (int i = t - 64; i < 2 * per; i += NT - 64) {
         const int v = i >= per ? 1 : 0;
         const int iv = i - v * per;
-        const int voff = v * 4 * n;
-        T *vec = v ? vB : vA;
         uint64_t *src;
         uint32_t *dst;
-        if (iv < nleft) {  // left neighbour's right edge: knot a-1-e = local 1-e
-          const int vi = iv / WPV, qq = iv - vi * WPV;
-          const int e = vi / n, rr = vi - e * n;
-          src = rec(type, rank - 1) + (kRecScalars + voff + 2 * n + e * n + rr) * WPV + qq;
-          dst = reinterpret_cast<uint32_t *>(vec + (1 - e) * nP + rr) + qq;
-        } else {  // right neighbour's left edge: knot b+e = local K+2+e
-          const int ii = iv - nleft;
-          const int vi = ii / WPV, qq = ii - vi * WPV;
-          const int e = vi / n, rr = vi - e * n;
-          src = rec(type, rank + 1) + (kRecScalars + voff + e * n + rr) * WPV + qq;
-          dst = reinterpret_cast<uint32_t *>(vec + (K + 2 + e) * nP + rr) + qq;
-        }
+        edge_slot(iv, nleft, type, v * 4 * n, v ? vB : vA, src, dst);
         uint32_t val = 0;
         if (xcd ? xwait(src, epoch, val) : xwait_lazy(src, epoch, val)) {
           *dst = val;

@@ -21,9 +21,12 @@ const void *rc_ptr_n(int n) {
 }
 }  // namespace
 
-// Fused recurrence: 256 and 512 threads; reference recurrence: 256 threads.
+// Fused recurrence: 64, 256 and 512 threads; reference recurrence: 64 and 256.
+// 64 threads: one wave per rank (two knots), no cross-wave reductions.
 const void *rowcast_kernel_ptr(int dtype, int n, int nt, bool fused) {
   const bool f64 = dtype == GBDPCG_F64;
+  if (nt == 64) return fused ? (f64 ? rc_ptr_n<double, 64, true>(n) : rc_ptr_n<float, 64, true>(n))
+                             : (f64 ? rc_ptr_n<double, 64, false>(n) : rc_ptr_n<float, 64, false>(n));
   if (!fused) {
     if (nt != 256) return nullptr;
     return f64 ? rc_ptr_n<double, 256, false>(n) : rc_ptr_n<float, 256, false>(n);

@@ -209,7 +209,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   if (tune_lookup(key, &hit)) {
     gbdpcg_config c = *cfg;
     c.knots_per_wg = hit.a;
-    c.block[0] = hit.b;
+    c.threads = hit.b;
     c.kernel = hit.c;
     int rc = gbdpcg_batch_create(out, dtype, systems, count, &c, 0);
     if (rc == GBDPCG_OK) {
@@ -234,7 +234,8 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
     maxN = std::max(maxN, systems[i].N);
   }
   const uint32_t caps[] = {0, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
-  const uint32_t def_nt = cfg->block[0] == 256 ? 256u : 512u;  // what the untuned plan would use
+  // what the untuned plan would use
+  const uint32_t def_nt = (cfg->threads == 256 || (cfg->threads == 0 && cfg->block[0] == 256)) ? 256u : 512u;
   std::vector<uint32_t> nts = {512};
   if (same_n) nts.push_back(256);
 
@@ -259,7 +260,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
       if (cap > maxN) continue;
       gbdpcg_config c = *cfg;
       c.knots_per_wg = cap;
-      c.block[0] = nt;
+      c.threads = nt;
       c.kernel = kid;
       gbdpcg_batch *p = nullptr;
       if (gbdpcg_batch_create(&p, dtype, systems, count, &c, 0) != GBDPCG_OK) continue;  // does not plan

@@ -46,7 +46,8 @@ class Config(ctypes.Structure):
     _fields_ = [("exit_tol", ctypes.c_double), ("max_iter", ctypes.c_uint32), ("grid", ctypes.c_uint32 * 3),
                 ("block", ctypes.c_uint32 * 3), ("empty_pinv", ctypes.c_int32), ("exit_mode", ctypes.c_int32),
                 ("precond", ctypes.c_int32), ("recurrence", ctypes.c_int32), ("stream", ctypes.c_void_p),
-                ("kernel", ctypes.c_uint32), ("knots_per_wg", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
+                ("kernel", ctypes.c_uint32), ("knots_per_wg", ctypes.c_uint32), ("threads", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
 
 
 class Result(ctypes.Structure):
@@ -184,8 +185,7 @@ def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS
     cfg.stream = stream
     cfg.kernel = int(kernel)
     cfg.knots_per_wg = int(knots_per_wg)
-    if threads:
-        cfg.block[0] = int(threads)
+    cfg.threads = int(threads)
     return cfg
 
 
@@ -211,11 +211,12 @@ def _dev(t, name):
 # ---------------------------------------------------------------------------
 
 def solve(S, gamma, lam, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, exit_mode=EXIT_ABS_ETA,
-          recurrence=RECUR_REFERENCE, precond=PRECOND_SS, wgs=0, stream=None, kernel=KERNEL_AUTO) -> Result:
+          recurrence=RECUR_REFERENCE, precond=PRECOND_SS, wgs=0, stream=None, kernel=KERNEL_AUTO,
+          threads=0) -> Result:
     """Solves S lam = gamma in place (lam holds the initial guess).  Waits for the
     stream; returns the result record."""
     dt = _dtype_code(S.dtype)
-    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel, threads=threads)
     res = Result()
     f = lib().gbdpcg_solve_device_f64 if dt == F64 else lib().gbdpcg_solve_device_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
@@ -225,21 +226,22 @@ def solve(S, gamma, lam, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, ex
 
 def solve_async(S, gamma, lam, n: int, N: int, Pinv=None, d_result=None, *, tol=1e-6, max_iter=25,
                 exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, precond=PRECOND_SS, wgs=0, stream=None,
-                kernel=KERNEL_AUTO) -> None:
+                kernel=KERNEL_AUTO, threads=0) -> None:
     """Launches a solve without waiting; the result record goes to d_result (a
     device uint8 tensor of RESULT_BYTES bytes) when given."""
     dt = _dtype_code(S.dtype)
-    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel, threads=threads)
     f = lib().gbdpcg_solve_device_async_f64 if dt == F64 else lib().gbdpcg_solve_device_async_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
             _dev(d_result, "d_result")))
 
 
 def adjoint(S, gamma, lam, glam, ggamma, gS, n: int, N: int, Pinv, *, forward=True, tol=1e-6, max_iter=25,
-            exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, wgs=0, stream=None, kernel=KERNEL_AUTO) -> Result:
+            exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, wgs=0, stream=None, kernel=KERNEL_AUTO,
+            threads=0) -> Result:
     """(Forward +) adjoint solve; writes ggamma and gS (cbtd).  Waits for the stream."""
     dt = _dtype_code(S.dtype)
-    cfg = make_config(tol, max_iter, exit_mode, recurrence, PRECOND_SS, wgs, _stream_handle(stream), kernel)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, PRECOND_SS, wgs, _stream_handle(stream), kernel, threads=threads)
     res = Result()
     f = lib().gbdpcg_adjoint_device_f64 if dt == F64 else lib().gbdpcg_adjoint_device_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma") if forward else None, _dev(lam, "lam"),

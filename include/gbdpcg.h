@@ -90,7 +90,9 @@ typedef struct gbdpcg_config {
   void *stream;       /* hipStream_t; NULL = the default stream                          */
   uint32_t kernel;    /* GBDPCG_KERNEL_*: 0 = automatic                                  */
   uint32_t knots_per_wg; /* cap on knots per workgroup, 0 = automatic (tuned or planner) */
-  uint32_t reserved[6];
+  uint32_t threads;   /* threads per workgroup of the register kernels: 64 (row broadcast
+                         only: one wave per rank), 256 or 512; 0 = automatic          */
+  uint32_t reserved[5];
 } gbdpcg_config;
 
 /* kernel selection (gbdpcg_config.kernel) */
@@ -291,8 +293,8 @@ int gbdpcg_tune_single(int dtype, uint32_t stateSize, uint32_t knotPoints, const
                        const void *d_gamma, const gbdpcg_config *cfg, uint32_t trials,
                        gbdpcg_tune_report *report);
 
-/* Batch: builds candidate plans of the same systems (cfg->knots_per_wg and
- * cfg->block[0] varied), runs each on the systems' own data and returns the
+/* Batch: builds candidate plans of the same systems (cfg->knots_per_wg,
+ * cfg->threads and the register layout varied), runs each on the systems' own data and returns the
  * fastest plan in *out.  The forward systems' lambda is restored to its value at
  * entry; adjoint outputs hold the last trial's values.  Not allowed during graph
  * capture.  trials = timed runs per candidate (0 = 3). */

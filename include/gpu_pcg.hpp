@@ -66,6 +66,7 @@ struct pcg_config {
   void *stream = nullptr;
   uint32_t kernel = GBDPCG_KERNEL_AUTO;
   uint32_t knots_per_wg = 0;    // 0 = automatic (tuned by gbdpcg_tune_single, else the planner)
+  uint32_t threads = 0;         // register kernels: 64 (row broadcast), 256 or 512; 0 = automatic
   gbdpcg_result last_result{};  // filled by the device-pointer solve
 
   pcg_config(T exit_tol = pcg_constants::DEFAULT_EPSILON<T>,
@@ -102,6 +103,7 @@ inline gbdpcg_config to_c(const pcg_config<T> *cfg) {
     c.stream = cfg->stream;
     c.kernel = cfg->kernel;
     c.knots_per_wg = cfg->knots_per_wg;
+    c.threads = cfg->threads;
   }
   return c;
 }

@@ -222,6 +222,35 @@ def test_forced_kernels(gpu, dtype, n, kernel, recurrence):
     assert rel(lam.cpu().numpy(), lam_ref) <= (1e-4 if dtype == np.float32 else 1e-10)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [12, 14])
+@pytest.mark.parametrize("N", [4, 10, 64, 128])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_one_wave_ranks(gpu, dtype, n, N, recurrence):
+    """Row-broadcast kernel at 64 threads: one wave per rank, two knots each (up
+    to 64 ranks), against the oracle like every forced layout.  Odd N does not
+    split into ranks of two knots: the request is declined."""
+    import gbdpcg
+    S9, g9 = lqr(n, 9, 9)
+    with pytest.raises(gbdpcg.GbdpcgError) as e:
+        gbdpcg.solve(dev(S9, dtype), dev(g9, dtype), dev(np.zeros(n * 9), dtype), n, 9, None, recurrence=recurrence,
+                     kernel=gbdpcg.KERNEL_ROWCAST, threads=64)
+    assert e.value.code == gbdpcg.ERR_UNSUPPORTED
+    S, g = lqr(n, N, 900 + n + N)
+    P = O.precond(S, n, N, 2)
+    Sd, Pd, gd = S.astype(dtype), P.astype(dtype), g.astype(dtype)
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    lam = dev(np.zeros(n * N), dtype)
+    r = gbdpcg.solve(dev(Sd, dtype), dev(gd, dtype), lam, n, N, dev(Pd, dtype), tol=tol, max_iter=400,
+                     exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence, kernel=gbdpcg.KERNEL_ROWCAST, threads=64)
+    assert r.status == gbdpcg.CONVERGED
+    _, it_same, *_ = O.pcg(Sd, Pd, gd, np.zeros(n * N, dtype), n, N, tol, 400, O.EXIT_REL, dtype,
+                           fused=recurrence == 1)
+    assert abs(r.iters - it_same) <= 1
+    lam_ref, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 0.0, r.iters)
+    assert rel(lam.cpu().numpy(), lam_ref) <= (1e-4 if dtype == np.float32 else 1e-10)
+
+
 def test_register_kernel_declines_unsupported(gpu):
     import gbdpcg
     n, N = 7, 16
