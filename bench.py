@@ -47,6 +47,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-workloads", action="store_true", help="skip the 512-system and n=28/N=512 workloads")
     ap.add_argument("--no-tune", action="store_true", help="time the planner's default layout, not the tuned one")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the other single-solve sizes (N = 32, 64, 256), e.g. for per-kernel profiles")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -156,7 +158,7 @@ def main():
     # the other single-solve sizes of the same problem family (n=14, m=7), tuned,
     # back to back from lambda0 = 0
     sweep = {}
-    for Ns in (32, 64, 128, 256):
+    for Ns in () if args.no_sweep else (32, 64, 128, 256):
         Ss_h, gs_h = gbdpcg.lqr_generate(n, m, Ns, 2000 + Ns, dtype=np.float32)
         Ss, gs = torch.from_numpy(Ss_h).to(dev), torch.from_numpy(gs_h).to(dev)
         Ps = torch.empty_like(Ss)
@@ -182,11 +184,18 @@ def main():
     achieved = hbm_bytes / (kern_us * 1e-6) / 1e9
     # measured HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
     # same command, scaled by the counters' reading of a plain copy of known size
+    # (the counters are looked up under the name of the kernel this layout runs)
+    layout = {"kernel": tune["kernel"] if tune else gbdpcg.KERNEL_REGISTER,
+              "threads": tune["threads"] if tune else 256, "wgs": tune["wgs"] if tune else None}
+    kname = {gbdpcg.KERNEL_LDS: "gbdpcg::pcg_kernel<float>",
+             gbdpcg.KERNEL_REGISTER: f"gbdpcg::pcg_reg_kernel<float, 14, {layout['threads']}, true>",
+             gbdpcg.KERNEL_ROWCAST: f"gbdpcg::pcg_rowcast_kernel<float, 14, {layout['threads']}, true>"}[layout["kernel"]]
+    layout["kernel_name"] = kname
     traffic = None
     counters = os.path.join(ROOT, "profiles", "pcg_reg_kernel_n14_N128_counters.json")
     if os.path.exists(counters):
         with open(counters) as f:
-            rec = json.load(f)["kernels"].get("gbdpcg::pcg_reg_kernel<float, 14, 256, true>")
+            rec = json.load(f)["kernels"].get(kname)
         if rec:
             traffic = int(rec["traffic_bytes"])
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -708,6 +708,25 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   const bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED && N >= 2;
   const size_t ts = tsize(dtype);
 
+  // a choice already made for this shape (this process, or the cache file) is kept
+  {
+    TuneChoice c;
+    Layout L;
+    if (tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
+        layout_from(dtype, n, N, c.a != 0, c.a == 2, (int)c.b, c.c, c.d, &L, fused) == GBDPCG_OK) {
+      if (report) {
+        gbdpcg_tune_report r{};
+        r.from_cache = 1;
+        r.knots_per_wg = ceil_div(N, L.G);
+        r.threads = (uint32_t)L.nt;
+        r.wgs = L.G;
+        r.kernel = L.bc ? GBDPCG_KERNEL_ROWCAST : L.reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+        *report = r;
+      }
+      return GBDPCG_OK;
+    }
+  }
+
   // candidates: the untuned default first, then every (kernel, threads, knots per
   // workgroup, XCD placement) that fits
   std::vector<Layout> cands;
