@@ -241,11 +241,16 @@ def main():
         torch.cuda.synchronize()
         batch_us = t_start.elapsed_time(t_end) * 1000.0 / nb
         rate = B / (batch_us * 1e-6)
+        e_lam = e_gg = 0.0
+        for d, (Sb64, _, gb64, gl64) in zip(systems, host64):
+            e_lam = max(e_lam, rel_err(np, d["lam"].cpu().numpy(), banded_cholesky_solve(np, Sb64, gb64, n, N)))
+            e_gg = max(e_gg, rel_err(np, d["ggamma"].cpu().numpy(), banded_cholesky_solve(np, Sb64, gl64, n, N)))
         batched = {"systems": B, "us_per_batch": round(batch_us, 2), "fwd_adj_solves_per_s": round(rate, 1),
                    "vs_back_to_back_plain": round(rate / (1e6 / us_per_solve), 2),
                    "mean_iters_fwd": float(np.mean([r.iters for r in bres])),
                    "mean_iters_adj": float(np.mean([r.adj_iters for r in bres])),
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
+                   "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg, "systems_checked": B},
                    "plan": plan.info(), "tune": plan.tune_report,
                    "cpu_baseline": cpu_batched(np, host64, n, N, tol, max_iter, args.cpu_seconds)}
 
@@ -265,22 +270,11 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # CPU baseline: FP64 host restatement of the reference iteration (oracle/), one thread
-    from oracle import oracle as O
-    S64, g64, P64 = S_h.astype(np.float64), g_h.astype(np.float64), P_h.astype(np.float64)
-    lam0 = np.zeros(n * N)
-    O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
-    cnt, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < args.cpu_seconds:
-        _, it_cpu, *_ = O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
-        cnt += 1
-    cpu_us = (time.perf_counter() - t0) * 1e6 / cnt
-    cpu = {"value": round(cpu_us, 2), "unit": "us/solve", "cores": 1, "kind": "port",
-           "sample": f"{cnt} FP64 solves of the same n=14,N=128 system (oracle/pcg_oracle.c, {it_cpu} iterations each)"}
-
-    # accuracy of the last timed solve against the FP64 oracle at the same iteration count
-    lam_ref, *_ = O.pcg(S64, P64, g64, lam0, n, N, 0.0, iters)
-    err = float(np.linalg.norm(lam_last - lam_ref) / np.linalg.norm(lam_ref))
+    # CPU baseline leg: the FP64 host restatement of the reference iteration
+    # (oracle/), one thread, which also checks the last timed solve
+    cpu, err = cpu_single(np, S_h, g_h, P_h, n, N, tol, max_iter, iters, lam_last, args.cpu_seconds)
+    # error against an FP64 dense Cholesky solve of the same system (numpy / scipy)
+    chol_err = dense_cholesky_error(np, S_h, g_h, n, N, lam_last)
 
     out = {
         "metric": METRIC, "value": round(us_per_solve, 3), "unit": "us/solve", "n_gpus": 1, "steps": K,
@@ -288,7 +282,7 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded LQR Schur complement, on-device SS Pinv)",
         "config": {"workload": "plain PCG single solve, n=14 m=7 N=128, symmetric-stair Pinv, rel resid 1e-5, "
                                "max_iter 173, lambda0=0", "n": n, "N": N, "recurrence": "fused (Chronopoulos-Gear)",
-                   "iters": iters, "rel_err_vs_fp64_oracle": err},
+                   "iters": iters, "rel_err_vs_fp64_oracle": err, "rel_err_vs_fp64_dense_cholesky": chol_err},
         "roofline": roofline, "cpu_baseline": cpu,
         "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
@@ -296,6 +290,65 @@ def main():
         "n14_sizes": sweep, "tune": tune, "batched": batched, "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
+
+
+def cpu_single(np, S_h, g_h, P_h, n, N, tol, max_iter, iters, lam_last, seconds):
+    """CPU baseline of the single solve: the FP64 host restatement of the
+    reference iteration (oracle/), one thread, timed on a bounded sample; as the
+    checker it also gives the last GPU solve's error against the FP64 iteration
+    run for the same number of iterations."""
+    from oracle import oracle as O
+    S64, g64, P64 = S_h.astype(np.float64), g_h.astype(np.float64), P_h.astype(np.float64)
+    lam0 = np.zeros(n * N)
+    O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
+    cnt, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        _, it_cpu, *_ = O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
+        cnt += 1
+    cpu_us = (time.perf_counter() - t0) * 1e6 / cnt
+    cpu = {"value": round(cpu_us, 2), "unit": "us/solve", "cores": 1, "kind": "port",
+           "sample": f"{cnt} FP64 solves of the same n=14,N=128 system (oracle/pcg_oracle.c, {it_cpu} iterations each)"}
+    lam_ref, *_ = O.pcg(S64, P64, g64, lam0, n, N, 0.0, iters)
+    err = float(np.linalg.norm(lam_last - lam_ref) / np.linalg.norm(lam_ref))
+    return cpu, err
+
+
+def banded_cholesky_solve(np, S_h, rhs, n, N):
+    """FP64 Cholesky solve of S x = rhs in banded storage (half-bandwidth 2n - 1:
+    the same factorisation as the dense one, restricted to the band), for the
+    larger configurations."""
+    from scipy.linalg import solveh_banded
+    B = np.asarray(S_h, dtype=np.float64).reshape(N, 3, n, n).transpose(0, 1, 3, 2)  # [k, block, row, column]
+    u = 2 * n - 1
+    ab = np.zeros((u + 1, n * N))
+    r, c = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    k = np.arange(N)[:, None]
+    up = r <= c  # diagonal blocks: their upper triangle
+    ab[(u + r[up] - c[up])[None, :], k * n + c[up][None, :]] = B[:, 1][:, up]
+    # blocks (k, k+1), k < N-1: wholly above the diagonal
+    if N > 1:
+        ab[(u + r - c - n).ravel()[None, :], (k[:-1] + 1) * n + c.ravel()[None, :]] = B[:-1, 2].reshape(N - 1, -1)
+    return solveh_banded(ab, np.asarray(rhs, dtype=np.float64))
+
+
+def rel_err(np, a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def dense_cholesky_error(np, S_h, g_h, n, N, lam):
+    """Relative error of lam against the FP64 dense Cholesky solution of S x = gamma
+    (S expanded from its cbtd blocks: block row k holds blocks (k, k-1), (k, k),
+    (k, k+1), each n x n column-major)."""
+    from scipy.linalg import cho_factor, cho_solve
+    B = S_h.astype(np.float64).reshape(N, 3, n, n)  # [k, block, column, row]
+    D = np.zeros((n * N, n * N))
+    for k in range(N):
+        for b in range(3):
+            j = k - 1 + b
+            if 0 <= j < N:
+                D[k * n:(k + 1) * n, j * n:(j + 1) * n] = B[k, b].T
+    x = cho_solve(cho_factor(D), g_h.astype(np.float64))
+    return float(np.linalg.norm(lam.astype(np.float64) - x) / np.linalg.norm(x))
 
 
 def cpu_batched(np, host, n, N, tol, max_iter, seconds):
@@ -358,7 +411,8 @@ def uneven_512(gbdpcg, torch, np, dev, autotune):
     rng = np.random.default_rng(3000)
     specs = [(int(rng.choice([6, 12, 14, 28])), int(rng.integers(16, 257))) for _ in range(512)]
     systems = []
-    for (n, N), (S_h, g_h) in zip(specs, _generate(gbdpcg, np, specs, 3000, np.float32)):
+    gen = _generate(gbdpcg, np, specs, 3000, np.float32)
+    for (n, N), (S_h, g_h) in zip(specs, gen):
         S = torch.from_numpy(S_h).to(dev)
         P = torch.empty_like(S)
         gbdpcg.precond_device(S, P, n, N, gbdpcg.PRECOND_SS)
@@ -373,9 +427,13 @@ def uneven_512(gbdpcg, torch, np, dev, autotune):
 
     us = _graph_time(torch, body, 10)
     res = plan.results()
+    err = max(rel_err(np, d["lam"].cpu().numpy(), banded_cholesky_solve(np, S_h, g_h, d["n"], d["N"]))
+              for d, (S_h, g_h) in zip(systems, gen))
     return {"systems": 512, "us_per_batch": round(us, 1), "solves_per_s": round(512 / (us * 1e-6), 1),
             "mean_iters": float(np.mean([r.iters for r in res])),
-            "all_converged": all(r.status == 0 for r in res), "plan": plan.info(), "tune": plan.tune_report}
+            "all_converged": all(r.status == 0 for r in res),
+            "max_rel_err_vs_fp64_cholesky": {"lambda": err, "systems_checked": 512},
+            "plan": plan.info(), "tune": plan.tune_report}
 
 
 def mpc_sweep(gbdpcg, torch, np, dev, dtype, count=256, n=28, N=512, steps=10):
@@ -399,11 +457,18 @@ def mpc_sweep(gbdpcg, torch, np, dev, dtype, count=256, n=28, N=512, steps=10):
 
     us = _graph_time(torch, body, 2)
     res = plan.results()
+    e_lam = e_gg = 0.0
+    for d in systems[::count // 16]:  # 16 systems, final state of the last re-solve
+        S64 = d["S"].cpu().numpy()
+        e_lam = max(e_lam, rel_err(np, d["lam"].cpu().numpy(), banded_cholesky_solve(np, S64, d["gamma"].cpu().numpy(), n, N)))
+        e_gg = max(e_gg, rel_err(np, d["ggamma"].cpu().numpy(), banded_cholesky_solve(np, S64, d["glam"].cpu().numpy(), n, N)))
     out = {"systems": count, "re_solves": steps, "ms_per_graph": round(us / 1000.0, 2),
            "fwd_adj_solves_per_s": round(count * steps / (us * 1e-6), 1),
            "last_mean_iters_fwd": float(np.mean([r.iters for r in res])),
            "last_mean_iters_adj": float(np.mean([r.adj_iters for r in res])),
-           "all_converged": all(r.status == 0 and r.adj_status == 0 for r in res), "plan": plan.info()}
+           "all_converged": all(r.status == 0 and r.adj_status == 0 for r in res),
+           "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg, "systems_checked": len(systems[::count // 16])},
+           "plan": plan.info()}
     plan.close()
     del systems
     torch.cuda.empty_cache()
