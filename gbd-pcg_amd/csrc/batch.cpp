@@ -34,12 +34,27 @@ using namespace gbdpcg;
     }                                                                                     \
   } while (0)
 
+// One persistent launch of a plan: a kernel class and its schedule.  A plan runs
+// its launches back to back on one stream (still one capturable call).
+struct BatchLaunch {
+  bool reg = false;    // register-resident kernel (else LDS-resident)
+  bool bc = false;     // ... in the row-broadcast layout
+  bool pair = false;   // ... with forward + adjoint systems iterated together
+  int kn = 0;          // stateSize specialisation (0 = runtime dispatch)
+  int nt = 256;        // threads per workgroup
+  uint32_t grid = 0, rounds = 0, occ = 0, items = 0, systems = 0;
+  size_t lds = 0;
+  size_t item_off = 0;  // offset of this launch's rounds x grid items in d_items
+  double makespan = 0.0;
+};
+
 struct gbdpcg_batch {
   int dtype = 0;
   int device = 0;
   uint32_t count = 0;
   std::vector<gbdpcg_system_desc> sys;
   std::vector<uint32_t> wgs;
+  std::vector<BatchLaunch> launches;
   KSys *d_sys = nullptr;
   uint32_t *d_items = nullptr;
   uint64_t *d_slots = nullptr;
@@ -47,20 +62,32 @@ struct gbdpcg_batch {
   int *d_err = nullptr;
   gbdpcg_batch_info info{};
   size_t slot_words = 0;
-  bool reg = false;  // register-resident kernel
-  bool bc = false;   // ... in the row-broadcast layout
-  bool pair = false; // ... with forward + adjoint systems iterated together
-  int kn = 0;        // its stateSize specialisation (0 = runtime dispatch)
-  int nt = 256;      // threads per workgroup
 };
 
-namespace gbdpcg {
-bool batch_uses_registers(const gbdpcg_batch *plan) { return plan && plan->reg; }
-int batch_kernel(const gbdpcg_batch *plan) {
-  if (!plan) return GBDPCG_KERNEL_AUTO;
-  return plan->bc ? GBDPCG_KERNEL_ROWCAST : plan->reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+namespace {
+// the launch holding the most systems (what the tuner reports)
+const BatchLaunch *primary(const gbdpcg_batch *plan) {
+  const BatchLaunch *b = nullptr;
+  for (const BatchLaunch &L : plan->launches)
+    if (!b || L.systems > b->systems) b = &L;
+  return b;
 }
-int batch_threads(const gbdpcg_batch *plan) { return plan ? plan->nt : 0; }
+}  // namespace
+
+namespace gbdpcg {
+bool batch_uses_registers(const gbdpcg_batch *plan) {
+  const BatchLaunch *b = plan ? primary(plan) : nullptr;
+  return b && b->reg;
+}
+int batch_kernel(const gbdpcg_batch *plan) {
+  const BatchLaunch *b = plan ? primary(plan) : nullptr;
+  if (!b) return GBDPCG_KERNEL_AUTO;
+  return b->bc ? GBDPCG_KERNEL_ROWCAST : b->reg ? GBDPCG_KERNEL_REGISTER : GBDPCG_KERNEL_LDS;
+}
+int batch_threads(const gbdpcg_batch *plan) {
+  const BatchLaunch *b = plan ? primary(plan) : nullptr;
+  return b ? b->nt : 0;
+}
 }  // namespace gbdpcg
 
 namespace {
@@ -98,6 +125,149 @@ double system_cost(const gbdpcg_system_desc &d, uint32_t G, bool fused, size_t t
   return passes * (iters * (exch + mv) + 2 * exch) + load + 3000.0;
 }
 
+// Plans one launch over the systems `idx` (all of one kernel class): workgroups
+// per system (G, fused), the launch's LDS size, co-resident grid and a gang list
+// schedule, whose items are appended to `items`.
+int plan_launch(int dtype, const gbdpcg_system_desc *systems, const std::vector<uint32_t> &idx,
+                const gbdpcg_config *cfg, const hipDeviceProp_t &prop, size_t budget, BatchLaunch &L,
+                std::vector<uint32_t> &G, std::vector<uint8_t> &fused, std::vector<uint32_t> &items) {
+  const size_t ts = tsize(dtype);
+  const bool fused_cfg = cfg->recurrence == GBDPCG_RECUR_FUSED;
+  const RegVariant rv{true, L.bc, L.pair};
+  size_t lds = 0;
+  // cfg->knots_per_wg caps the knots per workgroup (the autotuner's main knob)
+  const uint32_t kcap = cfg->knots_per_wg ? std::max<uint32_t>(cfg->knots_per_wg, 1u) : 0u;
+  for (uint32_t i : idx) {
+    const gbdpcg_system_desc &d = systems[i];
+    if (L.reg) {
+      const uint32_t kmax = reg_max_knots(dtype, d.n, L.nt, L.bc);
+      const uint32_t target = kcap ? std::min(kmax, std::max<uint32_t>(kcap, 2u)) : kmax;
+      uint32_t g = d.wgs;
+      if (!g) {  // the cap is a target: every workgroup keeps at least 2 knots
+        g = (d.N + target - 1) / target;
+        g = std::min(g, std::max<uint32_t>(d.N / 2, 1u));
+      }
+      if (g == 0 || (d.N + g - 1) / g > kmax || d.N / g < 2 || g > (uint32_t)kMaxWgsPerSystem) {
+        set_error("system %u: %u workgroups do not suit the register kernel (N=%u, 2..%u knots each)", i, g, d.N,
+                  kmax);
+        return GBDPCG_ERR_TOO_LARGE;
+      }
+      G[i] = g;
+      fused[i] = 1;
+      lds = std::max(lds, reg_lds_bytes(dtype, d.n, (d.N + g - 1) / g, L.nt, rv));
+    } else {
+      uint32_t kmax = max_knots(d.n, d.N, ts, budget);
+      if (kcap && kmax) kmax = std::min(kmax, kcap);
+      if (kmax == 0) {
+        set_error("system %u (n=%u) does not fit %zu B of LDS even with one knot per workgroup", i, d.n, budget);
+        return GBDPCG_ERR_TOO_LARGE;
+      }
+      uint32_t g = d.wgs ? d.wgs : (d.N + kmax - 1) / kmax;
+      if (g > d.N || (d.N + g - 1) / g > kmax) {
+        set_error("system %u: %u workgroups do not fit (N=%u, at most %u knots per workgroup)", i, g, d.N, kmax);
+        return GBDPCG_ERR_TOO_LARGE;
+      }
+      if (g > (uint32_t)kMaxWgsPerSystem) {
+        set_error("system %u needs %u workgroups (max %d)", i, g, kMaxWgsPerSystem);
+        return GBDPCG_ERR_TOO_LARGE;
+      }
+      fused[i] = fused_cfg && d.N / g >= 2;
+      G[i] = g;
+      lds = std::max(lds, item_lds_bytes(d.n, (d.N + g - 1) / g, g, ts));
+    }
+  }
+  lds = (lds + 255) & ~(size_t)255;
+  int occ = 0;
+  if (L.reg) {
+    HIPCHK(reg_prepare(dtype, L.kn, L.nt, lds, rv));
+    HIPCHK(reg_occupancy(dtype, L.kn, L.nt, lds, &occ, rv));
+  } else {
+    HIPCHK(pcg_prepare(dtype, lds));
+    HIPCHK(pcg_occupancy(dtype, lds, &occ));
+  }
+  if (occ <= 0) {
+    set_error("no workgroup fits at %zu B LDS", lds);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  const uint32_t grid = (uint32_t)prop.multiProcessorCount * (uint32_t)occ;
+  const uint32_t per_xcd = grid / kXcds;
+  for (uint32_t i : idx) {
+    if (G[i] > grid) {
+      set_error("system %u needs %u co-resident workgroups (capacity %u)", i, G[i], grid);
+      return GBDPCG_ERR_TOO_LARGE;
+    }
+  }
+
+  // gang list scheduling
+  std::vector<double> cost(idx.size());
+  for (size_t q = 0; q < idx.size(); ++q) cost[q] = system_cost(systems[idx[q]], G[idx[q]], fused[idx[q]], ts, L.pair);
+  std::vector<uint32_t> order(idx.size());
+  std::iota(order.begin(), order.end(), 0u);
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+  std::vector<double> free_at(grid, 0.0);
+  struct Item {
+    double start;
+    uint32_t sys, rank;
+  };
+  std::vector<std::vector<Item>> lists(grid);
+  std::vector<uint32_t> cand;
+  double makespan = 0.0;
+  for (uint32_t q : order) {
+    const uint32_t s = idx[q];
+    const uint32_t g = G[s];
+    std::vector<uint32_t> chosen;
+    double start = 0.0;
+    if (g <= per_xcd) {
+      double best = 1e300;
+      for (int x = 0; x < kXcds; ++x) {
+        cand.clear();
+        for (uint32_t j = 0; j < per_xcd; ++j) cand.push_back((uint32_t)x + (uint32_t)kXcds * j);
+        std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return free_at[a] < free_at[b]; });
+        const double st = free_at[cand[g - 1]];
+        if (st < best) {
+          best = st;
+          chosen.assign(cand.begin(), cand.begin() + g);
+        }
+      }
+      start = best;
+    } else {
+      cand.resize(grid);
+      std::iota(cand.begin(), cand.end(), 0u);
+      std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return free_at[a] < free_at[b]; });
+      chosen.assign(cand.begin(), cand.begin() + g);
+      start = free_at[cand[g - 1]];
+    }
+    // ranks in increasing workgroup index (neighbouring ranks stay close)
+    std::sort(chosen.begin(), chosen.end());
+    for (uint32_t r = 0; r < g; ++r) {
+      lists[chosen[r]].push_back({start, s, r});
+      free_at[chosen[r]] = start + cost[q];
+    }
+    makespan = std::max(makespan, start + cost[q]);
+  }
+  uint32_t rounds = 0, total_items = 0;
+  for (auto &l : lists) {
+    std::stable_sort(l.begin(), l.end(), [](const Item &a, const Item &b) {
+      return a.start < b.start || (a.start == b.start && a.sys < b.sys);
+    });
+    rounds = std::max<uint32_t>(rounds, (uint32_t)l.size());
+    total_items += (uint32_t)l.size();
+  }
+  L.item_off = items.size();
+  items.resize(items.size() + (size_t)rounds * grid, kNoItem);
+  for (uint32_t b = 0; b < grid; ++b)
+    for (size_t r = 0; r < lists[b].size(); ++r)
+      items[L.item_off + r * grid + b] = make_item(lists[b][r].sys, lists[b][r].rank);
+  L.grid = grid;
+  L.rounds = rounds;
+  L.occ = (uint32_t)occ;
+  L.lds = lds;
+  L.items = total_items;
+  L.systems = (uint32_t)idx.size();
+  L.makespan = makespan;
+  return GBDPCG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -131,171 +301,79 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   const size_t max_lds = prop.sharedMemPerBlock;
   const size_t budget = lds_budget_bytes ? std::min<size_t>(lds_budget_bytes, max_lds) : max_lds;
 
-  // kernel: register-resident rows when every system suits it (fused recurrence,
-  // stateSize in the specialised set, at least 4 knots), else LDS-resident rows
-  bool use_reg = fused_cfg && cfg->kernel != GBDPCG_KERNEL_LDS;
-  bool same_n = true;
-  bool f64_wide = false;  // FP64 rows of n = 28: only the 256-thread instance holds them
+  // Kernel classes, one launch each: register-resident rows of n <= 14 (one
+  // stateSize specialisation, or the runtime-n instance), rows of n = 28 (their
+  // own instance: they need more registers than the others), and the
+  // LDS-resident kernel for systems the register kernels do not take (other
+  // stateSizes, N < 4) or for every system when the config asks for it.
+  const bool reg_cfg = fused_cfg && cfg->kernel != GBDPCG_KERNEL_LDS;
+  std::vector<uint32_t> small, wide, lds_sys;
   for (uint32_t i = 0; i < count; ++i) {
-    if (!reg_supported((int)systems[i].n) || systems[i].N < 4) use_reg = false;
-    if (dtype == GBDPCG_F64 && systems[i].n > 14) f64_wide = true;
-    if (systems[i].n != systems[0].n) same_n = false;
+    const gbdpcg_system_desc &d = systems[i];
+    if (reg_cfg && reg_supported((int)d.n) && d.N >= 4)
+      (d.n > 14 ? wide : small).push_back(i);
+    else
+      lds_sys.push_back(i);
   }
-  if (f64_wide && !same_n) use_reg = false;
-  if (cfg->kernel == GBDPCG_KERNEL_REGISTER && !use_reg) {
+  if (cfg->kernel == GBDPCG_KERNEL_REGISTER && !lds_sys.empty()) {
     set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28} and N >= 4 for every system");
     return GBDPCG_ERR_UNSUPPORTED;
   }
+  auto same_n = [&](const std::vector<uint32_t> &v) {
+    for (uint32_t i : v)
+      if (systems[i].n != systems[v[0]].n) return false;
+    return true;
+  };
   // row-broadcast layout (on request; the autotuner tries it): one stateSize in
   // {12, 14} for every system
-  const bool bc = use_reg && same_n && reg_supported((int)systems[0].n, true) &&
-                  cfg->kernel == GBDPCG_KERNEL_ROWCAST;
-  if (cfg->kernel == GBDPCG_KERNEL_ROWCAST && !bc) {
+  const bool bc = cfg->kernel == GBDPCG_KERNEL_ROWCAST;
+  if (bc && (!lds_sys.empty() || !wide.empty() || !same_n(small) || !reg_supported((int)systems[0].n, true))) {
     set_error("row-broadcast kernel needs the fused recurrence, one n in {12,14} and N >= 4 for every system");
     return GBDPCG_ERR_UNSUPPORTED;
   }
-  // forward + adjoint systems run both solves in one loop (the paired kernel;
-  // other systems of the plan run one pass after the other on it as usual)
-  // on rows of n <= 14
-  bool any_pair = false, small_n = true;
-  for (uint32_t i = 0; i < count; ++i) {
-    if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) any_pair = true;
-    if (systems[i].n > 14) small_n = false;
-  }
-  const bool pair = use_reg && any_pair && small_n;
-  const RegVariant rv{true, bc, pair};
-  // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads only
-  const bool only256 = f64_wide || (bc && dtype == GBDPCG_F64);
   // threads per workgroup: cfg->threads (or the reference-style block[0]) = 256
-  // asks for 256 when every system shares n
+  // asks for 256 when a launch's systems share n
   const bool want256 = cfg->threads == 256 || (cfg->threads == 0 && cfg->block[0] == 256);
-  const int nt = (use_reg && same_n && (want256 || only256)) ? 256 : 512;
-  const int kn = same_n ? (int)systems[0].n : 0;
 
-  // workgroups per system and the launch's LDS size
-  std::vector<uint32_t> G(count);
-  std::vector<uint8_t> fused(count);
-  size_t lds = 0;
-  // cfg->knots_per_wg caps the knots per workgroup (the autotuner's main knob)
-  const uint32_t kcap = cfg->knots_per_wg ? std::max<uint32_t>(cfg->knots_per_wg, 1u) : 0u;
-  for (uint32_t i = 0; use_reg && i < count; ++i) {
-    const gbdpcg_system_desc &d = systems[i];
-    const uint32_t kmax = reg_max_knots(dtype, d.n, nt, bc);
-    const uint32_t target = kcap ? std::min(kmax, std::max<uint32_t>(kcap, 2u)) : kmax;
-    uint32_t g = d.wgs;
-    if (!g) {  // the cap is a target: every workgroup keeps at least 2 knots
-      g = (d.N + target - 1) / target;
-      g = std::min(g, std::max<uint32_t>(d.N / 2, 1u));
-    }
-    if (g == 0 || (d.N + g - 1) / g > kmax || d.N / g < 2 || g > (uint32_t)kMaxWgsPerSystem) {
-      set_error("system %u: %u workgroups do not suit the register kernel (N=%u, 2..%u knots each)", i, g, d.N,
-                kmax);
-      return GBDPCG_ERR_TOO_LARGE;
-    }
-    G[i] = g;
-    fused[i] = 1;
-    lds = std::max(lds, reg_lds_bytes(dtype, d.n, (d.N + g - 1) / g, nt, rv));
+  std::vector<BatchLaunch> launches;
+  std::vector<std::vector<uint32_t>> members;
+  if (!small.empty()) {
+    BatchLaunch L;
+    L.reg = true;
+    L.bc = bc;
+    const bool one_n = same_n(small);
+    L.kn = one_n ? (int)systems[small[0]].n : 0;
+    // forward + adjoint systems run both solves in one loop (the paired kernel;
+    // other systems of the launch run one pass after the other on it)
+    for (uint32_t i : small)
+      if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
+    // FP64 whole rows (row broadcast): 256 threads only
+    L.nt = (one_n && (want256 || (bc && dtype == GBDPCG_F64))) ? 256 : 512;
+    launches.push_back(L);
+    members.push_back(small);
   }
-  for (uint32_t i = 0; !use_reg && i < count; ++i) {
-    const gbdpcg_system_desc &d = systems[i];
-    uint32_t kmax = max_knots(d.n, d.N, ts, budget);
-    if (kcap && kmax) kmax = std::min(kmax, kcap);
-    if (kmax == 0) {
-      set_error("system %u (n=%u) does not fit %zu B of LDS even with one knot per workgroup", i, d.n, budget);
-      return GBDPCG_ERR_TOO_LARGE;
-    }
-    uint32_t g = d.wgs ? d.wgs : (d.N + kmax - 1) / kmax;
-    if (g > d.N || (d.N + g - 1) / g > kmax) {
-      set_error("system %u: %u workgroups do not fit (N=%u, at most %u knots per workgroup)", i, g, d.N, kmax);
-      return GBDPCG_ERR_TOO_LARGE;
-    }
-    fused[i] = fused_cfg && d.N / g >= 2;
-    if (g > (uint32_t)kMaxWgsPerSystem) {
-      set_error("system %u needs %u workgroups (max %d)", i, g, kMaxWgsPerSystem);
-      return GBDPCG_ERR_TOO_LARGE;
-    }
-    G[i] = g;
-    lds = std::max(lds, item_lds_bytes(d.n, (d.N + g - 1) / g, g, ts));
+  if (!wide.empty()) {
+    BatchLaunch L;
+    L.reg = true;
+    L.kn = 28;
+    // FP64 rows of n = 28: only the 256-thread instance holds them
+    L.nt = (want256 || dtype == GBDPCG_F64) ? 256 : 512;
+    launches.push_back(L);
+    members.push_back(wide);
   }
-  lds = (lds + 255) & ~(size_t)255;
-  int occ = 0;
-  if (use_reg) {
-    HIPCHK(reg_prepare(dtype, kn, nt, lds, rv));
-    HIPCHK(reg_occupancy(dtype, kn, nt, lds, &occ, rv));
-  } else {
-    HIPCHK(pcg_prepare(dtype, lds));
-    HIPCHK(pcg_occupancy(dtype, lds, &occ));
-  }
-  if (occ <= 0) {
-    set_error("no workgroup fits at %zu B LDS", lds);
-    return GBDPCG_ERR_TOO_LARGE;
-  }
-  const uint32_t grid = (uint32_t)prop.multiProcessorCount * (uint32_t)occ;
-  const uint32_t per_xcd = grid / kXcds;
-  for (uint32_t i = 0; i < count; ++i) {
-    if (G[i] > grid) {
-      set_error("system %u needs %u co-resident workgroups (capacity %u)", i, G[i], grid);
-      return GBDPCG_ERR_TOO_LARGE;
-    }
+  if (!lds_sys.empty()) {
+    BatchLaunch L;
+    L.nt = kThreads;
+    launches.push_back(L);
+    members.push_back(lds_sys);
   }
 
-  // gang list scheduling
-  std::vector<double> cost(count);
-  for (uint32_t i = 0; i < count; ++i) cost[i] = system_cost(systems[i], G[i], fused[i], ts, pair);
-  std::vector<uint32_t> order(count);
-  std::iota(order.begin(), order.end(), 0u);
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
-  std::vector<double> free_at(grid, 0.0);
-  struct Item {
-    double start;
-    uint32_t sys, rank;
-  };
-  std::vector<std::vector<Item>> lists(grid);
-  std::vector<uint32_t> idx;
-  double makespan = 0.0;
-  for (uint32_t s : order) {
-    const uint32_t g = G[s];
-    std::vector<uint32_t> chosen;
-    double start = 0.0;
-    if (g <= per_xcd) {
-      double best = 1e300;
-      for (int x = 0; x < kXcds; ++x) {
-        idx.clear();
-        for (uint32_t j = 0; j < per_xcd; ++j) idx.push_back((uint32_t)x + (uint32_t)kXcds * j);
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return free_at[a] < free_at[b]; });
-        const double st = free_at[idx[g - 1]];
-        if (st < best) {
-          best = st;
-          chosen.assign(idx.begin(), idx.begin() + g);
-        }
-      }
-      start = best;
-    } else {
-      idx.resize(grid);
-      std::iota(idx.begin(), idx.end(), 0u);
-      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return free_at[a] < free_at[b]; });
-      chosen.assign(idx.begin(), idx.begin() + g);
-      start = free_at[idx[g - 1]];
-    }
-    // ranks in increasing workgroup index (neighbouring ranks stay close)
-    std::sort(chosen.begin(), chosen.end());
-    for (uint32_t r = 0; r < g; ++r) {
-      lists[chosen[r]].push_back({start, s, r});
-      free_at[chosen[r]] = start + cost[s];
-    }
-    makespan = std::max(makespan, start + cost[s]);
+  std::vector<uint32_t> G(count, 0), items;
+  std::vector<uint8_t> fused(count, 0);
+  for (size_t li = 0; li < launches.size(); ++li) {
+    const int rc = plan_launch(dtype, systems, members[li], cfg, prop, budget, launches[li], G, fused, items);
+    if (rc < 0) return rc;
   }
-  uint32_t rounds = 0, total_items = 0;
-  for (auto &l : lists) {
-    std::stable_sort(l.begin(), l.end(), [](const Item &a, const Item &b) {
-      return a.start < b.start || (a.start == b.start && a.sys < b.sys);
-    });
-    rounds = std::max<uint32_t>(rounds, (uint32_t)l.size());
-    total_items += (uint32_t)l.size();
-  }
-  std::vector<uint32_t> items((size_t)rounds * grid, kNoItem);
-  for (uint32_t b = 0; b < grid; ++b)
-    for (size_t r = 0; r < lists[b].size(); ++r) items[r * grid + b] = make_item(lists[b][r].sys, lists[b][r].rank);
 
   // device state
   auto plan = new gbdpcg_batch();
@@ -304,6 +382,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   plan->count = count;
   plan->sys.assign(systems, systems + count);
   plan->wgs = G;
+  plan->launches = launches;
   std::vector<size_t> slot_off(count);
   size_t words = 0;
   const uint32_t wpv = (uint32_t)(ts / 4);
@@ -359,18 +438,19 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     set_error("gbdpcg_batch_create: upload failed: %s", hipGetErrorString(e));
     return fail(GBDPCG_ERR_DEVICE);
   }
-  plan->info.grid = grid;
-  plan->info.rounds = rounds;
-  plan->info.wgs_per_cu = (uint32_t)occ;
-  plan->info.lds_bytes = (uint32_t)lds;
-  plan->reg = use_reg;
-  plan->bc = bc;
-  plan->pair = pair;
-  plan->kn = kn;
-  plan->nt = use_reg ? nt : kThreads;
-  plan->info.systems = count;
-  plan->info.total_items = total_items;
-  plan->info.est_cost = makespan;
+  // summary over the launches: widest grid, rounds one after the other
+  gbdpcg_batch_info &in = plan->info;
+  in.wgs_per_cu = ~0u;
+  for (const BatchLaunch &L : launches) {
+    in.grid = std::max(in.grid, L.grid);
+    in.rounds += L.rounds;
+    in.wgs_per_cu = std::min(in.wgs_per_cu, L.occ);
+    in.lds_bytes = std::max(in.lds_bytes, (uint32_t)L.lds);
+    in.total_items += L.items;
+    in.est_cost += L.makespan;
+  }
+  in.systems = count;
+  in.launches = (uint32_t)launches.size();
   *out = plan;
   return GBDPCG_OK;
 }
@@ -382,14 +462,16 @@ int gbdpcg_batch_run(gbdpcg_batch *plan, void *stream) {
   }
   KSys dummy;
   memset(&dummy, 0, sizeof dummy);
-  if (plan->reg) {
-    HIPCHK(reg_launch(plan->dtype, plan->kn, plan->nt, dummy, plan->d_sys, plan->d_items, plan->info.rounds, 1,
-                      plan->info.grid, plan->info.lds_bytes, static_cast<hipStream_t>(stream),
-                      RegVariant{true, plan->bc, plan->pair}));
-    return GBDPCG_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  for (const BatchLaunch &L : plan->launches) {
+    if (L.reg) {
+      HIPCHK(reg_launch(plan->dtype, L.kn, L.nt, dummy, plan->d_sys, plan->d_items + L.item_off, L.rounds, 1, L.grid,
+                        L.lds, st, RegVariant{true, L.bc, L.pair}));
+    } else {
+      HIPCHK(pcg_launch(plan->dtype, dummy, plan->d_sys, plan->d_items + L.item_off, L.rounds, 1, L.grid, L.lds,
+                        st));
+    }
   }
-  HIPCHK(pcg_launch(plan->dtype, dummy, plan->d_sys, plan->d_items, plan->info.rounds, 1, plan->info.grid,
-                    plan->info.lds_bytes, static_cast<hipStream_t>(stream)));
   return GBDPCG_OK;
 }
 

@@ -1341,12 +1341,13 @@ __device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *ld
       default: break;  // the planner only admits the sizes above
     }
   } else {
+    // batch plans give rows of n = 28 a launch (and instance) of their own: the
+    // runtime-n instance then needs the registers of n <= 14 only
     switch (d.n) {
       case 2: reg_item<T, 2, NT, FUSED>(d, rank, lds); break;
       case 6: reg_item<T, 6, NT, FUSED>(d, rank, lds); break;
       case 12: reg_item<T, 12, NT, FUSED>(d, rank, lds); break;
       case 14: reg_item<T, 14, NT, FUSED>(d, rank, lds); break;
-      case 28: reg_item<T, 28, NT, FUSED>(d, rank, lds); break;
       default: break;  // the planner only admits the sizes above
     }
   }
@@ -1375,9 +1376,18 @@ __device__ __forceinline__ void reg_entry(const KSys &one, const KSys *__restric
   }
 }
 
+// Waves per SIMD the column-chunk kernel is compiled for: the FP32 runtime-n
+// instance at 512 threads (rows of n <= 14) is held to 128 VGPRs, so that two
+// workgroups share a CU in batch plans (the specialised ones fit by themselves).
+template <typename T, int NS, int NT>
+constexpr int reg_waves_per_eu() {
+  return (sizeof(T) == 4 && NT == 512 && NS == 0) ? 4 : 1;
+}
+
 // Register-resident rows, reg_lpr(n) lanes per row, column chunks.
 template <typename T, int NS, int NT, bool FUSED>
-__global__ void __launch_bounds__(NT) pcg_reg_kernel(KSys one, const KSys *__restrict__ systems,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(reg_waves_per_eu<T, NS, NT>())))
+pcg_reg_kernel(KSys one, const KSys *__restrict__ systems,
                                                      const uint32_t *__restrict__ items, uint32_t rounds,
                                                      uint32_t stride) {
   reg_entry<T, NS, NT, FUSED, false>(one, systems, items, rounds, stride);

@@ -227,17 +227,13 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   }
 
   // candidate knobs: kernel x threads per workgroup x knots-per-workgroup caps
-  bool same_n = true;
+  // (identical plans are timed once, see the deduplication below)
   uint32_t maxN = 0;
-  for (uint32_t i = 0; i < count; ++i) {
-    if (systems[i].n != systems[0].n) same_n = false;
-    maxN = std::max(maxN, systems[i].N);
-  }
+  for (uint32_t i = 0; i < count; ++i) maxN = std::max(maxN, systems[i].N);
   const uint32_t caps[] = {0, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
   // what the untuned plan would use
   const uint32_t def_nt = (cfg->threads == 256 || (cfg->threads == 0 && cfg->block[0] == 256)) ? 256u : 512u;
-  std::vector<uint32_t> nts = {512};
-  if (same_n) nts.push_back(256);
+  const std::vector<uint32_t> nts = {def_nt, def_nt == 512 ? 256u : 512u};
 
   LambdaSnapshot snap;
   int rc = snap.save(dtype, systems, count, st);

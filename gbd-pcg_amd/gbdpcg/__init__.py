@@ -70,7 +70,7 @@ class SystemDesc(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     _fields_ = [("grid", ctypes.c_uint32), ("rounds", ctypes.c_uint32), ("wgs_per_cu", ctypes.c_uint32),
                 ("lds_bytes", ctypes.c_uint32), ("systems", ctypes.c_uint32), ("total_items", ctypes.c_uint32),
-                ("est_cost", ctypes.c_double)]
+                ("est_cost", ctypes.c_double), ("launches", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class TuneReport(ctypes.Structure):
@@ -431,7 +431,7 @@ class Batch:
     def info(self) -> dict:
         bi = BatchInfo()
         check(lib().gbdpcg_batch_info_get(self._h, ctypes.byref(bi)))
-        return {f: getattr(bi, f) for f, _ in bi._fields_}
+        return {f: getattr(bi, f) for f, _ in bi._fields_ if f != "reserved"}
 
     def wgs(self):
         out = (ctypes.c_uint32 * self.count)()

@@ -235,14 +235,18 @@ typedef struct gbdpcg_system_desc {
 
 typedef struct gbdpcg_batch gbdpcg_batch;
 
+/* A plan runs one persistent launch per kernel class present (register rows of
+ * n <= 14, rows of n = 28, LDS-resident rows for the others), back to back. */
 typedef struct gbdpcg_batch_info {
-  uint32_t grid;           /* workgroups launched                                  */
-  uint32_t rounds;         /* items per workgroup (max)                            */
-  uint32_t wgs_per_cu;     /* co-resident workgroups per CU at this LDS size        */
-  uint32_t lds_bytes;      /* dynamic LDS per workgroup                             */
+  uint32_t grid;           /* workgroups of the widest launch                       */
+  uint32_t rounds;         /* items per workgroup, summed over the launches         */
+  uint32_t wgs_per_cu;     /* co-resident workgroups per CU (smallest launch value)  */
+  uint32_t lds_bytes;      /* dynamic LDS per workgroup (largest launch value)      */
   uint32_t systems;
   uint32_t total_items;
-  double est_cost;         /* planner's makespan estimate (arbitrary units)         */
+  double est_cost;         /* planner's makespan estimate, summed (arbitrary units) */
+  uint32_t launches;       /* persistent launches per gbdpcg_batch_run              */
+  uint32_t reserved;
 } gbdpcg_batch_info;
 
 /* cfg supplies exit_mode, recurrence and stream; tolerance/max_iter are per system.

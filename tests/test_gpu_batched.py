@@ -71,6 +71,28 @@ def test_uneven_batch(gpu, dtype, recurrence):
     check(systems, host, b.results(), dtype, recurrence)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_kernel_classes_in_one_plan(gpu, dtype, adjoint):
+    """A plan with rows of n <= 14, rows of n = 28 and a stateSize the register
+    kernels do not take (n = 7) runs three launches back to back (fused
+    recurrence): every system against the oracle, forward and adjoint."""
+    import gbdpcg
+    tol = 1e-5 if dtype == np.float32 else 1e-9
+    specs = [(6, 24, tol), (14, 40, tol), (28, 20, tol), (7, 18, tol), (14, 9, tol), (28, 33, tol), (7, 5, tol)]
+    systems, host = make_batch(specs, dtype, adjoint=adjoint, seed0=9700)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+    info = b.info()
+    assert info["launches"] == 3 and info["systems"] == len(specs)
+    b.run()
+    check(systems, host, b.results(), dtype, 1, adjoint=adjoint)
+    # the register kernel cannot take n = 7: forcing it is declined
+    with pytest.raises(gbdpcg.GbdpcgError) as e:
+        gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
+                     kernel=gbdpcg.KERNEL_REGISTER)
+    assert e.value.code == gbdpcg.ERR_UNSUPPORTED
+
+
 @pytest.mark.parametrize("recurrence", [0, 1])
 def test_forward_adjoint_batch(gpu, recurrence):
     import gbdpcg
