@@ -301,12 +301,16 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
   const size_t max_lds = prop.sharedMemPerBlock;
   const size_t budget = lds_budget_bytes ? std::min<size_t>(lds_budget_bytes, max_lds) : max_lds;
 
-  // Kernel classes, one launch each: register-resident rows of n <= 14 (one
-  // stateSize specialisation, or the runtime-n instance), rows of n = 28 (their
-  // own instance: they need more registers than the others), and the
-  // LDS-resident kernel for systems the register kernels do not take (other
-  // stateSizes, N < 4) or for every system when the config asks for it.
+  // Kernel choice.  Register-resident rows take the fused recurrence, the
+  // specialised stateSizes and N >= 4; the LDS-resident kernel takes the rest.
+  // By default a plan is one persistent launch: the register kernel when every
+  // system suits it (the runtime-n instance when stateSizes differ), else the
+  // LDS kernel for all.  cfg->batch_split asks for one launch per kernel class
+  // present instead: rows of n <= 14 (their runtime-n instance then needs the
+  // registers of those sizes only: two 512-thread workgroups per CU), rows of
+  // n = 28, and the LDS kernel for systems the register kernels do not take.
   const bool reg_cfg = fused_cfg && cfg->kernel != GBDPCG_KERNEL_LDS;
+  const bool split = cfg->batch_split != 0;
   std::vector<uint32_t> small, wide, lds_sys;
   for (uint32_t i = 0; i < count; ++i) {
     const gbdpcg_system_desc &d = systems[i];
@@ -315,15 +319,31 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     else
       lds_sys.push_back(i);
   }
-  if (cfg->kernel == GBDPCG_KERNEL_REGISTER && !lds_sys.empty()) {
-    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28} and N >= 4 for every system");
-    return GBDPCG_ERR_UNSUPPORTED;
-  }
   auto same_n = [&](const std::vector<uint32_t> &v) {
     for (uint32_t i : v)
       if (systems[i].n != systems[v[0]].n) return false;
     return true;
   };
+  if (!split) {
+    // one launch: FP64 rows of n = 28 need the 256-thread instance, which has no
+    // runtime-n form, so mixed FP64 plans holding them go to the LDS kernel
+    std::vector<uint32_t> all(count);
+    std::iota(all.begin(), all.end(), 0u);
+    const bool f64_wide_mixed = dtype == GBDPCG_F64 && !wide.empty() && !same_n(all);
+    if (lds_sys.empty() && !f64_wide_mixed) {
+      small = all;
+      wide.clear();
+    } else {
+      lds_sys = all;
+      small.clear();
+      wide.clear();
+    }
+  }
+  if (cfg->kernel == GBDPCG_KERNEL_REGISTER && !lds_sys.empty()) {
+    set_error("register kernel needs the fused recurrence, n in {2,6,12,14,28} and N >= 4 for every system%s",
+              split ? "" : " (FP64 plans mixing n = 28 with other sizes: set batch_split)");
+    return GBDPCG_ERR_UNSUPPORTED;
+  }
   // row-broadcast layout (on request; the autotuner tries it): one stateSize in
   // {12, 14} for every system
   const bool bc = cfg->kernel == GBDPCG_KERNEL_ROWCAST;
@@ -342,13 +362,19 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     L.reg = true;
     L.bc = bc;
     const bool one_n = same_n(small);
-    L.kn = one_n ? (int)systems[small[0]].n : 0;
-    // forward + adjoint systems run both solves in one loop (the paired kernel;
-    // other systems of the launch run one pass after the other on it)
+    bool has_wide = false;
+    for (uint32_t i : small)
+      if (systems[i].n > 14) has_wide = true;
+    // runtime-n instances: 0 = every specialised size, -1 = sizes up to 14
+    L.kn = one_n ? (int)systems[small[0]].n : has_wide ? 0 : -1;
+    // forward + adjoint systems of n <= 14 run both solves in one loop (the
+    // paired kernel; other systems of the launch run one pass after the other)
     for (uint32_t i : small)
       if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
-    // FP64 whole rows (row broadcast): 256 threads only
-    L.nt = (one_n && (want256 || (bc && dtype == GBDPCG_F64))) ? 256 : 512;
+    if (has_wide) L.pair = false;
+    // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads only
+    const bool only256 = dtype == GBDPCG_F64 && (bc || has_wide);
+    L.nt = (one_n && (want256 || only256)) ? 256 : 512;
     launches.push_back(L);
     members.push_back(small);
   }
@@ -356,7 +382,6 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     BatchLaunch L;
     L.reg = true;
     L.kn = 28;
-    // FP64 rows of n = 28: only the 256-thread instance holds them
     L.nt = (want256 || dtype == GBDPCG_F64) ? 256 : 512;
     launches.push_back(L);
     members.push_back(wide);

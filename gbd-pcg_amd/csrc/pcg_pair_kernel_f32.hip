@@ -26,7 +26,10 @@ const void *pp_n(int n, bool bc) {
     }
   }
   switch (n) {
-    case 0: if constexpr (NT == 512) return pp<0, NT, false>(); return nullptr;
+    case 0:
+    case -1:  // mixed sizes (paired plans hold rows of n <= 14 only)
+      if constexpr (NT == 512) return pp<0, NT, false>();
+      return nullptr;
     case 2: return pp<2, NT, false>();
     case 6: return pp<6, NT, false>();
     case 12: return pp<12, NT, false>();

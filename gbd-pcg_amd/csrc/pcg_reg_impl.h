@@ -1328,11 +1328,13 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
 
 }  // namespace
 
-// NS != 0: the kernel is specialised for stateSize NS.  NS == 0: runtime
-// dispatch over the specialised sizes (batches of uneven stateSize).
+// NS > 0: the kernel is specialised for stateSize NS.  Runtime dispatch for
+// batches of uneven stateSize: NS == 0 over every specialised size, NS == -1
+// over those up to 14 (plans that give rows of n = 28 a launch of their own;
+// the instance then needs the registers of n <= 14 only).
 template <typename T, int NS, int NT, bool FUSED, bool BC>
 __device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *lds) {
-  if constexpr (NS != 0) {
+  if constexpr (NS > 0) {
     reg_item<T, NS, NT, FUSED, BC>(d, rank, lds);
   } else if constexpr (BC) {
     switch (d.n) {
@@ -1341,13 +1343,14 @@ __device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *ld
       default: break;  // the planner only admits the sizes above
     }
   } else {
-    // batch plans give rows of n = 28 a launch (and instance) of their own: the
-    // runtime-n instance then needs the registers of n <= 14 only
     switch (d.n) {
       case 2: reg_item<T, 2, NT, FUSED>(d, rank, lds); break;
       case 6: reg_item<T, 6, NT, FUSED>(d, rank, lds); break;
       case 12: reg_item<T, 12, NT, FUSED>(d, rank, lds); break;
       case 14: reg_item<T, 14, NT, FUSED>(d, rank, lds); break;
+      case 28:
+        if constexpr (NS == 0) reg_item<T, 28, NT, FUSED>(d, rank, lds);
+        break;
       default: break;  // the planner only admits the sizes above
     }
   }
@@ -1377,11 +1380,12 @@ __device__ __forceinline__ void reg_entry(const KSys &one, const KSys *__restric
 }
 
 // Waves per SIMD the column-chunk kernel is compiled for: the FP32 runtime-n
-// instance at 512 threads (rows of n <= 14) is held to 128 VGPRs, so that two
-// workgroups share a CU in batch plans (the specialised ones fit by themselves).
+// instance for rows of n <= 14 (NS = -1) at 512 threads is held to 128 VGPRs,
+// so that two workgroups share a CU in batch plans (the specialised ones fit
+// by themselves).
 template <typename T, int NS, int NT>
 constexpr int reg_waves_per_eu() {
-  return (sizeof(T) == 4 && NT == 512 && NS == 0) ? 4 : 1;
+  return (sizeof(T) == 4 && NT == 512 && NS == -1) ? 4 : 1;
 }
 
 // Register-resident rows, reg_lpr(n) lanes per row, column chunks.
@@ -1414,7 +1418,7 @@ __device__ __forceinline__ void pair_item(const KSys &d, uint32_t rank, T *lds) 
 
 template <typename T, int NS, int NT, bool BC>
 __device__ __forceinline__ void pair_dispatch(const KSys &d, uint32_t rank, T *lds) {
-  if constexpr (NS != 0) {
+  if constexpr (NS > 0) {
     pair_item<T, NS, NT, BC>(d, rank, lds);
   } else if constexpr (BC) {
     switch (d.n) {
@@ -1434,12 +1438,13 @@ __device__ __forceinline__ void pair_dispatch(const KSys &d, uint32_t rank, T *l
 }
 
 // Waves per SIMD the paired kernel is compiled for: FP32 column-chunk rows of
-// n <= 14 at 512 threads are held to 128 VGPRs, so that two workgroups share a
+// n <= 14 (and the runtime-n instance, which holds those sizes only) at 512
+// threads are held to 128 VGPRs, so that two workgroups share a
 // CU (a batch of 64 systems of N = 128 then runs in one round); the compiler
 // spills what does not fit.
 template <typename T, int NS, int NT, bool BC>
 constexpr int pair_waves_per_eu() {
-  return (sizeof(T) == 4 && NT == 512 && !BC && NS != 0 && NS <= 14) ? 4 : 1;
+  return (sizeof(T) == 4 && NT == 512 && !BC && NS <= 14) ? 4 : 1;
 }
 
 // Forward + adjoint solves iterated together (see reg_item_pair); same launch

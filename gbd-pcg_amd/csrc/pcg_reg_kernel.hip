@@ -24,8 +24,11 @@ const void *reg_ptr_n(int n) {
     case 12: return reg_ptr<T, 12, NT>();
     case 14: return reg_ptr<T, 14, NT>();
     case 28: return reg_ptr<T, 28, NT>();
-    case 0:  // mixed sizes (batches): 512 threads only
+    case 0:  // mixed sizes (batches), every specialised size: 512 threads only
       if constexpr (NT == 512) return reg_ptr<T, 0, NT>();
+      return nullptr;
+    case -1:  // mixed sizes up to 14 (batches whose n = 28 rows run apart): 512 only
+      if constexpr (NT == 512) return reg_ptr<T, -1, NT>();
       return nullptr;
     default: return nullptr;
   }

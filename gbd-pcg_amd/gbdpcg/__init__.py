@@ -47,7 +47,7 @@ class Config(ctypes.Structure):
                 ("block", ctypes.c_uint32 * 3), ("empty_pinv", ctypes.c_int32), ("exit_mode", ctypes.c_int32),
                 ("precond", ctypes.c_int32), ("recurrence", ctypes.c_int32), ("stream", ctypes.c_void_p),
                 ("kernel", ctypes.c_uint32), ("knots_per_wg", ctypes.c_uint32), ("threads", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 5)]
+                ("batch_split", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
 
 
 class Result(ctypes.Structure):
@@ -173,7 +173,7 @@ def _dtype_code(dt) -> int:
 def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS_ETA,
                 recurrence: int = RECUR_REFERENCE, precond: int = PRECOND_SS, wgs: int = 0,
                 stream: Optional[int] = None, kernel: int = KERNEL_AUTO, knots_per_wg: int = 0,
-                threads: int = 0) -> Config:
+                threads: int = 0, batch_split: bool = False) -> Config:
     cfg = Config()
     lib().gbdpcg_config_init(ctypes.byref(cfg))
     cfg.exit_tol = float(tol)
@@ -186,6 +186,7 @@ def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS
     cfg.kernel = int(kernel)
     cfg.knots_per_wg = int(knots_per_wg)
     cfg.threads = int(threads)
+    cfg.batch_split = 1 if batch_split else 0
     return cfg
 
 
@@ -383,7 +384,8 @@ class Batch:
 
     def __init__(self, systems: Sequence[dict], *, exit_mode=EXIT_REL_RESID, recurrence=RECUR_REFERENCE,
                  lds_budget: int = 0, tol=1e-5, max_iter=173, knots_per_wg: int = 0, threads: int = 0,
-                 kernel: int = KERNEL_AUTO, autotune: bool = False, trials: int = 3, stream=None):
+                 kernel: int = KERNEL_AUTO, autotune: bool = False, trials: int = 3, stream=None,
+                 split_launches: bool = False):
         if not systems:
             raise ValueError("empty batch")
         self._keep = list(systems)
@@ -401,7 +403,7 @@ class Batch:
             d.flags = (SYS_FORWARD if s.get("forward", True) else 0) | (SYS_ADJOINT if s.get("adjoint", False) else 0)
             d.wgs = int(s.get("wgs", 0))
         cfg = make_config(exit_mode=exit_mode, recurrence=recurrence, stream=_stream_handle(stream), kernel=kernel,
-                          knots_per_wg=knots_per_wg, threads=threads)
+                          knots_per_wg=knots_per_wg, threads=threads, batch_split=split_launches)
         h = ctypes.c_void_p()
         self.tune_report = None
         if autotune:

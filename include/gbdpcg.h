@@ -92,7 +92,10 @@ typedef struct gbdpcg_config {
   uint32_t knots_per_wg; /* cap on knots per workgroup, 0 = automatic (tuned or planner) */
   uint32_t threads;   /* threads per workgroup of the register kernels: 64 (row broadcast
                          only: one wave per rank), 256 or 512; 0 = automatic          */
-  uint32_t reserved[5];
+  uint32_t batch_split; /* batch plans: 0 = one persistent launch for every system;
+                           1 = one launch per kernel class present (rows of n <= 14,
+                           rows of n = 28, LDS-resident rows), back to back          */
+  uint32_t reserved[4];
 } gbdpcg_config;
 
 /* kernel selection (gbdpcg_config.kernel) */
@@ -235,8 +238,9 @@ typedef struct gbdpcg_system_desc {
 
 typedef struct gbdpcg_batch gbdpcg_batch;
 
-/* A plan runs one persistent launch per kernel class present (register rows of
- * n <= 14, rows of n = 28, LDS-resident rows for the others), back to back. */
+/* A plan runs one persistent launch, or with cfg->batch_split one per kernel
+ * class present (register rows of n <= 14, rows of n = 28, LDS-resident rows for
+ * the others), back to back. */
 typedef struct gbdpcg_batch_info {
   uint32_t grid;           /* workgroups of the widest launch                       */
   uint32_t rounds;         /* items per workgroup, summed over the launches         */

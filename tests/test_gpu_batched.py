@@ -73,24 +73,39 @@ def test_uneven_batch(gpu, dtype, recurrence):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("adjoint", [False, True])
-def test_kernel_classes_in_one_plan(gpu, dtype, adjoint):
-    """A plan with rows of n <= 14, rows of n = 28 and a stateSize the register
-    kernels do not take (n = 7) runs three launches back to back (fused
-    recurrence): every system against the oracle, forward and adjoint."""
+@pytest.mark.parametrize("split", [False, True])
+def test_kernel_classes_in_one_plan(gpu, dtype, adjoint, split):
+    """Rows of n <= 14, rows of n = 28 and a stateSize the register kernels do
+    not take (n = 7) in one plan (fused recurrence): one persistent launch (the
+    LDS kernel for all), or with split_launches one launch per kernel class,
+    back to back.  Every system against the oracle, forward and adjoint; then
+    the same plan without n = 7."""
     import gbdpcg
     tol = 1e-5 if dtype == np.float32 else 1e-9
     specs = [(6, 24, tol), (14, 40, tol), (28, 20, tol), (7, 18, tol), (14, 9, tol), (28, 33, tol), (7, 5, tol)]
     systems, host = make_batch(specs, dtype, adjoint=adjoint, seed0=9700)
-    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+    kw = dict(exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, split_launches=split)
+    b = gbdpcg.Batch(systems, **kw)
     info = b.info()
-    assert info["launches"] == 3 and info["systems"] == len(specs)
+    assert info["launches"] == (3 if split else 1) and info["systems"] == len(specs)
     b.run()
     check(systems, host, b.results(), dtype, 1, adjoint=adjoint)
     # the register kernel cannot take n = 7: forcing it is declined
     with pytest.raises(gbdpcg.GbdpcgError) as e:
-        gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
-                     kernel=gbdpcg.KERNEL_REGISTER)
+        gbdpcg.Batch(systems, kernel=gbdpcg.KERNEL_REGISTER, **kw)
     assert e.value.code == gbdpcg.ERR_UNSUPPORTED
+    keep = [i for i, s in enumerate(specs) if s[0] != 7]
+    sub, sub_host = [systems[i] for i in keep], [host[i] for i in keep]
+    for d in sub:
+        for key in ("lam", "ggamma", "gS"):
+            if key in d:
+                d[key].zero_()
+    b2 = gbdpcg.Batch(sub, **kw)
+    # one launch for everything, except FP64 mixing n = 28 with smaller rows
+    # (no runtime-n instance holds FP64 rows of n = 28: the LDS kernel runs it)
+    assert b2.info()["launches"] == (2 if split else 1)
+    b2.run()
+    check(sub, sub_host, b2.results(), dtype, 1, adjoint=adjoint)
 
 
 @pytest.mark.parametrize("recurrence", [0, 1])
