@@ -125,6 +125,10 @@ double system_cost(const gbdpcg_system_desc &d, uint32_t G, bool fused, size_t t
   return passes * (iters * (exch + mv) + 2 * exch) + load + 3000.0;
 }
 
+// Paired instances for rows of n = 28 (one lane per row): FP32 at 512 threads,
+// FP64 at 256.
+bool pair28(int dtype, int nt) { return nt == (dtype == GBDPCG_F64 ? 256 : 512); }
+
 // Plans one launch over the systems `idx` (all of one kernel class): workgroups
 // per system (G, fused), the launch's LDS size, co-resident grid and a gang list
 // schedule, whose items are appended to `items`.
@@ -367,14 +371,15 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
       if (systems[i].n > 14) has_wide = true;
     // runtime-n instances: 0 = every specialised size, -1 = sizes up to 14
     L.kn = one_n ? (int)systems[small[0]].n : has_wide ? 0 : -1;
-    // forward + adjoint systems of n <= 14 run both solves in one loop (the
-    // paired kernel; other systems of the launch run one pass after the other)
-    for (uint32_t i : small)
-      if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
-    if (has_wide) L.pair = false;
     // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads only
     const bool only256 = dtype == GBDPCG_F64 && (bc || has_wide);
     L.nt = (one_n && (want256 || only256)) ? 256 : 512;
+    // forward + adjoint systems run both solves in one loop (the paired kernel;
+    // other systems of the launch run one pass after the other on it): rows of
+    // n <= 14, and rows of n = 28 on their one-lane-per-row instances
+    for (uint32_t i : small)
+      if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
+    if (has_wide && !(one_n && pair28(dtype, L.nt))) L.pair = false;
     launches.push_back(L);
     members.push_back(small);
   }
@@ -383,6 +388,9 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     L.reg = true;
     L.kn = 28;
     L.nt = (want256 || dtype == GBDPCG_F64) ? 256 : 512;
+    for (uint32_t i : wide)
+      if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
+    L.pair = L.pair && pair28(dtype, L.nt);
     launches.push_back(L);
     members.push_back(wide);
   }

@@ -1,6 +1,6 @@
 // Paired forward + adjoint register-resident kernel, float instances (the kernel
 // is in pcg_reg_impl.h): both register layouts, stateSize 2, 6, 12, 14 (and
-// mixed sizes at 512 threads), 256 and 512 threads.
+// mixed sizes at 512 threads), 256 and 512 threads; stateSize 28 at 512.
 #include "pcg_reg_impl.h"
 
 namespace gbdpcg {
@@ -34,6 +34,9 @@ const void *pp_n(int n, bool bc) {
     case 6: return pp<6, NT, false>();
     case 12: return pp<12, NT, false>();
     case 14: return pp<14, NT, false>();
+    case 28:  // one lane per row: 512 threads only
+      if constexpr (NT == 512) return pp<28, NT, false>();
+      return nullptr;
     default: return nullptr;
   }
 }

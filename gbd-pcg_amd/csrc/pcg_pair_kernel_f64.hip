@@ -1,6 +1,7 @@
 // Paired forward + adjoint register-resident kernel, double instances (the kernel
 // is in pcg_reg_impl.h): both register layouts, stateSize 2, 6, 12, 14 (and
-// mixed sizes at 512 threads), 256 and 512 threads (whole rows: 256 only).
+// mixed sizes at 512 threads), 256 and 512 threads (whole rows: 256 only);
+// stateSize 28 at 256.
 #include "pcg_reg_impl.h"
 
 namespace gbdpcg {
@@ -34,6 +35,9 @@ const void *pp_n(int n, bool bc) {
     case 6: return pp<6, NT, false>();
     case 12: return pp<12, NT, false>();
     case 14: return pp<14, NT, false>();
+    case 28:  // one lane per row: 256 threads only
+      if constexpr (NT == 256) return pp<28, NT, false>();
+      return nullptr;
     default: return nullptr;
   }
 }

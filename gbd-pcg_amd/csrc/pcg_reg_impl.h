@@ -1427,7 +1427,7 @@ __device__ __forceinline__ void pair_dispatch(const KSys &d, uint32_t rank, T *l
       default: break;
     }
   } else {
-    switch (d.n) {  // paired plans hold rows of n <= 14 only
+    switch (d.n) {  // runtime-n paired plans hold rows of n <= 14 only
       case 2: pair_item<T, 2, NT, false>(d, rank, lds); break;
       case 6: pair_item<T, 6, NT, false>(d, rank, lds); break;
       case 12: pair_item<T, 12, NT, false>(d, rank, lds); break;
