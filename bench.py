@@ -189,7 +189,8 @@ def main():
               "threads": tune["threads"] if tune else 256, "wgs": tune["wgs"] if tune else None}
     kname = {gbdpcg.KERNEL_LDS: "gbdpcg::pcg_kernel<float>",
              gbdpcg.KERNEL_REGISTER: f"gbdpcg::pcg_reg_kernel<float, 14, {layout['threads']}, true>",
-             gbdpcg.KERNEL_ROWCAST: f"gbdpcg::pcg_rowcast_kernel<float, 14, {layout['threads']}, true>"}[layout["kernel"]]
+             gbdpcg.KERNEL_ROWCAST: f"gbdpcg::pcg_rowcast_kernel<float, 14, {layout['threads']}, true>"}[
+        layout["kernel"]]
     layout["kernel_name"] = kname
     traffic = None
     counters = os.path.join(ROOT, "profiles", "pcg_reg_kernel_n14_N128_counters.json")
@@ -474,14 +475,17 @@ def mpc_sweep(gbdpcg, torch, np, dev, dtype, count=256, n=28, N=512, steps=10):
     e_lam = e_gg = 0.0
     for d in systems[::count // 16]:  # 16 systems, final state of the last re-solve
         S64 = d["S"].cpu().numpy()
-        e_lam = max(e_lam, rel_err(np, d["lam"].cpu().numpy(), banded_cholesky_solve(np, S64, d["gamma"].cpu().numpy(), n, N)))
-        e_gg = max(e_gg, rel_err(np, d["ggamma"].cpu().numpy(), banded_cholesky_solve(np, S64, d["glam"].cpu().numpy(), n, N)))
+        x = banded_cholesky_solve(np, S64, d["gamma"].cpu().numpy(), n, N)
+        y = banded_cholesky_solve(np, S64, d["glam"].cpu().numpy(), n, N)
+        e_lam = max(e_lam, rel_err(np, d["lam"].cpu().numpy(), x))
+        e_gg = max(e_gg, rel_err(np, d["ggamma"].cpu().numpy(), y))
     out = {"systems": count, "re_solves": steps, "ms_per_graph": round(us / 1000.0, 2),
            "fwd_adj_solves_per_s": round(count * steps / (us * 1e-6), 1),
            "last_mean_iters_fwd": float(np.mean([r.iters for r in res])),
            "last_mean_iters_adj": float(np.mean([r.adj_iters for r in res])),
            "all_converged": all(r.status == 0 and r.adj_status == 0 for r in res),
-           "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg, "systems_checked": len(systems[::count // 16])},
+           "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg,
+                                             "systems_checked": len(systems[::count // 16])},
            "plan": plan.info()}
     plan.close()
     del systems

@@ -1,6 +1,7 @@
-// Register-resident PCG kernel for gfx950: the latency path (templates; the
-// fused-recurrence instances are compiled in pcg_reg_kernel.hip, the
-// reference-recurrence ones in pcg_reg_ref_kernel.hip).
+// Register-resident PCG kernels for gfx950: the latency path (templates,
+// instantiated in pcg_reg_kernel.hip (fused recurrence, column chunks),
+// pcg_reg_ref_kernel.hip (reference recurrence), pcg_rowcast_kernel.hip (row
+// broadcast) and pcg_pair_kernel_f32/_f64.hip (forward + adjoint together)).
 //
 // Same method and exchange protocol as pcg_kernel.hip, with every block row a
 // workgroup owns held in registers instead of LDS.  FUSED: the single-reduction
@@ -9,10 +10,12 @@
 // p'v and the edge knots of v = S p, the second r'z and r'r only (z = Pinv r is
 // formed on one ghost knot each side from the resident ghost rows of Pinv, so
 // p needs no halo).  Fused form:
-//   * LPR lanes (reg_lpr: 1 for n <= 2, else 2) share one matrix row; each lane
-//     keeps a CW-column chunk of that row of Pinv (rows of the owned knots plus
-//     one ghost knot each side) and of S (owned knots) in VGPRs for the whole
-//     solve, loaded from HBM once;
+//   * LPR lanes (reg_lpr: 2 for most sizes, 1 for n <= 2 and for the n = 28
+//     layouts that hold whole rows) share one matrix row; each lane keeps a
+//     CW-column chunk of that row of Pinv (rows of the owned knots plus one
+//     ghost knot each side) and of S (owned knots) in VGPRs for the whole
+//     solve, loaded from HBM once.  Row-broadcast layout (BC): one lane per row,
+//     16 lanes per knot, whole rows, the vector window broadcast by DPP;
 //   * per iteration: one pass u = Pinv r (owned + ghost rows), one pass w = S u
 //     (owned rows), then one cross-workgroup exchange carrying (r'u, w'u, r'r)
 //     and the two edge knots of w each side;

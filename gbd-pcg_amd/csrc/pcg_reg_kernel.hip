@@ -41,7 +41,8 @@ const void *reg_kernel(int dtype, int n, int nt, RegVariant v) {
   }
   if (v.bc) return rowcast_kernel_ptr(dtype, n, nt, v.fused);
   if (!v.fused) return reg_ref_kernel_ptr(dtype, n, nt);
-  if (dtype == GBDPCG_F64) return nt == 512 ? reg_ptr_n<double, 512>(n) : nt == 256 ? reg_ptr_n<double, 256>(n) : nullptr;
+  if (dtype == GBDPCG_F64)
+    return nt == 512 ? reg_ptr_n<double, 512>(n) : nt == 256 ? reg_ptr_n<double, 256>(n) : nullptr;
   return nt == 512 ? reg_ptr_n<float, 512>(n) : nt == 256 ? reg_ptr_n<float, 256>(n) : nullptr;
 }
 }  // namespace

@@ -217,7 +217,8 @@ def solve(S, gamma, lam, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, ex
     """Solves S lam = gamma in place (lam holds the initial guess).  Waits for the
     stream; returns the result record."""
     dt = _dtype_code(S.dtype)
-    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel, threads=threads)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel,
+                      threads=threads)
     res = Result()
     f = lib().gbdpcg_solve_device_f64 if dt == F64 else lib().gbdpcg_solve_device_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
@@ -231,7 +232,8 @@ def solve_async(S, gamma, lam, n: int, N: int, Pinv=None, d_result=None, *, tol=
     """Launches a solve without waiting; the result record goes to d_result (a
     device uint8 tensor of RESULT_BYTES bytes) when given."""
     dt = _dtype_code(S.dtype)
-    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel, threads=threads)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel,
+                      threads=threads)
     f = lib().gbdpcg_solve_device_async_f64 if dt == F64 else lib().gbdpcg_solve_device_async_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
             _dev(d_result, "d_result")))
@@ -242,7 +244,8 @@ def adjoint(S, gamma, lam, glam, ggamma, gS, n: int, N: int, Pinv, *, forward=Tr
             threads=0) -> Result:
     """(Forward +) adjoint solve; writes ggamma and gS (cbtd).  Waits for the stream."""
     dt = _dtype_code(S.dtype)
-    cfg = make_config(tol, max_iter, exit_mode, recurrence, PRECOND_SS, wgs, _stream_handle(stream), kernel, threads=threads)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, PRECOND_SS, wgs, _stream_handle(stream), kernel,
+                      threads=threads)
     res = Result()
     f = lib().gbdpcg_adjoint_device_f64 if dt == F64 else lib().gbdpcg_adjoint_device_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma") if forward else None, _dev(lam, "lam"),
