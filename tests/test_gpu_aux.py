@@ -224,12 +224,13 @@ def test_forced_kernels(gpu, dtype, n, kernel, recurrence):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("n", [12, 14])
-@pytest.mark.parametrize("N", [4, 10, 64, 128])
+@pytest.mark.parametrize("N", [4, 10, 64, 128, 256])
 @pytest.mark.parametrize("recurrence", [0, 1])
 def test_one_wave_ranks(gpu, dtype, n, N, recurrence):
-    """Row-broadcast kernel at 64 threads: one wave per rank, two knots each (up
-    to 64 ranks), against the oracle like every forced layout.  Odd N does not
-    split into ranks of two knots: the request is declined."""
+    """Row-broadcast kernel at 64 threads: one wave per rank, two knots each (128
+    ranks at N = 256: lanes poll a second rank each after the first), against the
+    oracle like every forced layout.  Odd N does not split into ranks of two
+    knots: the request is declined."""
     import gbdpcg
     S9, g9 = lqr(n, 9, 9)
     with pytest.raises(gbdpcg.GbdpcgError) as e:
