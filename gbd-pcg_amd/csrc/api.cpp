@@ -708,8 +708,11 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
   const bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED && N >= 2;
   const size_t ts = tsize(dtype);
 
+  // the cache holds the choice for automatic solves: read and written only when
+  // the config leaves the kernel and the threads per workgroup open
+  const bool open_cfg = cfg->kernel == GBDPCG_KERNEL_AUTO && cfg->threads == 0;
   // a choice already made for this shape (this process, or the cache file) is kept
-  {
+  if (open_cfg) {
     TuneChoice c;
     Layout L;
     if (tune_lookup(single_key(dev, dtype, n, N, fused, cfg->exit_mode), &c) &&
@@ -866,8 +869,9 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     return GBDPCG_ERR_TIMEOUT;
   }
   const Layout &B = cands[best];
-  tune_store(single_key(dev, dtype, n, N, fused, cfg->exit_mode),
-             TuneChoice{B.bc ? 2u : B.reg ? 1u : 0u, (uint32_t)B.nt, B.G, B.stride});
+  if (open_cfg)
+    tune_store(single_key(dev, dtype, n, N, fused, cfg->exit_mode),
+               TuneChoice{B.bc ? 2u : B.reg ? 1u : 0u, (uint32_t)B.nt, B.G, B.stride});
   if (report) {
     gbdpcg_tune_report r{};
     r.candidates = timed;

@@ -203,7 +203,8 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   gbdpcg_tune_report rep{};
   const std::string key = "batch:" + device_tag(dev) + ":" + std::to_string(dtype) + ":" +
                           std::to_string(cfg->recurrence) + ":" + std::to_string(cfg->exit_mode) + ":" +
-                          std::to_string(cfg->kernel) + ":" + std::to_string(count) + ":" +
+                          std::to_string(cfg->kernel) + ":" + std::to_string(cfg->threads) + ":" +
+                          std::to_string(cfg->batch_split) + ":" + std::to_string(count) + ":" +
                           std::to_string(systems_hash(systems, count));
   TuneChoice hit;
   if (tune_lookup(key, &hit)) {

@@ -297,8 +297,9 @@ typedef struct gbdpcg_tune_report {
  * the caller's data is not modified) and makes the fastest the default for later
  * solves of this shape on this device whose cfg leaves grid[0], knots_per_wg,
  * threads and kernel automatic.  A choice already cached for the shape is
- * returned without timing (report->from_cache = 1).  trials = timed runs per
- * candidate (0 = 5). */
+ * returned without timing (report->from_cache = 1).  With cfg->kernel or
+ * cfg->threads set, only the matching candidates are timed and the cache is
+ * neither read nor written.  trials = timed runs per candidate (0 = 5). */
 int gbdpcg_tune_single(int dtype, uint32_t stateSize, uint32_t knotPoints, const void *d_S, const void *d_Pinv,
                        const void *d_gamma, const gbdpcg_config *cfg, uint32_t trials,
                        gbdpcg_tune_report *report);
