@@ -6,11 +6,13 @@
   generator's own A, B, Q, R;
 * the host preconditioner matches the oracle's restatement of the definition;
 * CSR -> cbtd conversion, including rejection of entries outside the pattern;
-* argument validation returns error codes (and never needs the device).
+* argument validation returns error codes (and never needs the device);
+* the ctypes mirrors of the C structs have the header's sizes and field offsets.
 """
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -168,3 +170,30 @@ def test_argument_validation_without_device():
     d = (gbdpcg.SystemDesc * 1)()
     assert L.gbdpcg_batch_create(ctypes.byref(h), gbdpcg.F32, d, 1, ctypes.byref(cfg), 0) == gbdpcg.ERR_INVALID
     assert L.gbdpcg_batch_destroy(None) == 0
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes Structure of the binding against a C program compiled with the
+    header: size and the offset of every field."""
+    structs = {"gbdpcg_config": gbdpcg.Config, "gbdpcg_result": gbdpcg.Result,
+               "gbdpcg_system_desc": gbdpcg.SystemDesc, "gbdpcg_batch_info": gbdpcg.BatchInfo,
+               "gbdpcg_tune_report": gbdpcg.TuneReport}
+    c_name = {"lam": "lambda"}  # Python keyword in the mirror
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "gbdpcg.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in py._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {c_name.get(field, field)}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, key, val = line.split()
+        got[(cname, key)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for field, _ in py._fields_:
+            assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
