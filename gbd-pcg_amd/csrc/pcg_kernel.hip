@@ -36,9 +36,7 @@ namespace gbdpcg {
 // result.  Contains one __syncthreads.
 template <typename T>
 __device__ __forceinline__ void block_sum3(T &a, T &b, T &c, T *scratch) {
-  a = wave_sum(a);
-  b = wave_sum(b);
-  c = wave_sum(c);
+  wave_sum3(a, b, c);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     scratch[w * 3 + 0] = a;

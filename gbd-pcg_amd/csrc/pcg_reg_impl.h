@@ -256,9 +256,7 @@ struct RegItem {
       acc1 += word_value(w2, 1);
       acc2 += word_value(w2, 2);
     }
-    acc0 = wave_sum(acc0);
-    acc1 = wave_sum(acc1);
-    acc2 = wave_sum(acc2);
+    wave_sum3(acc0, acc1, acc2);
     if (t == 0) {
       tot[0] = acc0;
       tot[1] = acc1;
@@ -301,9 +299,7 @@ struct RegItem {
         acc1 += word_value(wv_, 1);
         acc2 += word_value(wv_, 2);
       }
-      acc0 = wave_sum(acc0);
-      acc1 = wave_sum(acc1);
-      acc2 = wave_sum(acc2);
+      wave_sum3(acc0, acc1, acc2);
       if (t == 0) {
         tot[0] = acc0;
         tot[1] = acc1;
@@ -398,8 +394,8 @@ struct RegItem {
 #pragma unroll
         for (int l = 0; l < 6; ++l) acc[l] += word_value(wv_, l);
       }
-#pragma unroll
-      for (int l = 0; l < 6; ++l) acc[l] = wave_sum(acc[l]);
+      wave_sum3(acc[0], acc[1], acc[2]);
+      wave_sum3(acc[3], acc[4], acc[5]);
       if (t == 0) {
 #pragma unroll
         for (int l = 0; l < 6; ++l) tot[l] = acc[l];
