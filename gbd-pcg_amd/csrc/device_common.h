@@ -122,7 +122,8 @@ __device__ __forceinline__ bool xwait_words(uint64_t *p, uint32_t epoch, uint32_
 // a / b for the scalar recurrences.  FP32: hardware reciprocal (1 ulp) and one
 // residual correction of the quotient, about 1 ulp in all (no denormal-mode
 // switches, unlike the IEEE sequence, and four dependent operations instead of
-// ten); b must be finite and non-zero, which the callers check first.
+// ten).  Meaningful for finite non-zero b only: callers test b and discard the
+// quotient otherwise.
 template <typename T>
 __device__ __forceinline__ T qdiv(T a, T b) {
   if constexpr (sizeof(T) == 4) {

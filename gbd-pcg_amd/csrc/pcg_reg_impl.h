@@ -869,6 +869,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       {
         uint32_t iter = 0;
         const uint32_t max_iter = d.max_iter;
+        T last_eta = eta, last_rr = rr;  // of the last exchange (uniform)
         while (iter < max_iter) {
           // x += alpha p ; r -= alpha s  (own in registers, ghosts by the ghost lanes)
           if (it.owner) {
@@ -900,30 +901,31 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
           // scalar recurrence below
           const T w_g = it.ghost ? it.wv[it.gpos] : (T)0;
           ++iter;
-          out.eta = (double)eta_new;
-          out.rr = (double)rr;
-          if (it.rel ? ((double)rr <= thr) : (__builtin_fabs((double)eta_new) < thr)) {
-            out.status = GBDPCG_CONVERGED;
-            break;
-          }
-          if (!finite_t(eta_new)) {
-            out.status = GBDPCG_NONFINITE;
-            break;
-          }
-          // beta and eta'/alpha are independent: two divisions in flight at once
+          last_eta = eta_new;
+          last_rr = rr;
+          // the stop tests and the next coefficients are independent: all are
+          // computed first, then one branch (a shorter dependency chain; the
+          // coefficients of a stopping iteration are discarded).  beta and
+          // eta'/alpha: two divisions in flight at once.
+          const bool conv = it.rel ? ((double)rr <= thr) : (__builtin_fabs((double)eta_new) < thr);
+          const bool nonfin = !finite_t(eta_new);
           const T beta = qdiv(eta_new, eta);
           const T den = delta_new - beta * qdiv(eta_new, alpha);
-          if (den == (T)0 || !finite_t(den)) {
-            out.status = GBDPCG_BREAKDOWN;
+          const bool brk = den == (T)0 || !finite_t(den);
+          const T alpha_new = qdiv(eta_new, den);
+          if (conv || nonfin || brk) {
+            out.status = conv ? GBDPCG_CONVERGED : nonfin ? GBDPCG_NONFINITE : GBDPCG_BREAKDOWN;
             break;
           }
-          alpha = qdiv(eta_new, den);
+          alpha = alpha_new;
           eta = eta_new;
           p_i = fmaT(beta, p_i, u_i);
           s_i = fmaT(beta, s_i, w_i);
           if (it.ghost) s_g = fmaT(beta, s_g, w_g);
         }
         out.iters = (int)iter;
+        out.eta = (double)last_eta;
+        out.rr = (double)last_rr;
       }
     }
   finish:
