@@ -192,15 +192,21 @@ def main():
              gbdpcg.KERNEL_ROWCAST: f"gbdpcg::pcg_rowcast_kernel<float, 14, {layout['threads']}, true>"}[
         layout["kernel"]]
     layout["kernel_name"] = kname
-    traffic = None
+    traffic, traffic_kernel = None, None
     counters = os.path.join(ROOT, "profiles", "pcg_reg_kernel_n14_N128_counters.json")
     if os.path.exists(counters):
         with open(counters) as f:
-            rec = json.load(f)["kernels"].get(kname)
-        if rec:
-            traffic = int(rec["traffic_bytes"])
+            kernels = json.load(f)["kernels"]
+        # the same kernel at the other thread count when this layout was not the
+        # profiled one (the tuner's choice between them is within noise)
+        family = kname.split("<")[0] + "<float, 14, "
+        for name in [kname] + sorted(k for k in kernels if k.startswith(family) and k.endswith(", true>")):
+            if name in kernels:
+                traffic, traffic_kernel = int(kernels[name]["traffic_bytes"]), name
+                break
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "algorithmic_bytes": hbm_bytes,
+                "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_kernel": traffic_kernel,
+                "algorithmic_bytes": hbm_bytes,
                 "note": "latency-bound: one cross-workgroup exchange per iteration; see per_iteration_us"}
 
     # batched forward + adjoint (64 systems, same shape), one persistent launch
