@@ -48,6 +48,18 @@ struct RegOut {
   double eta, rr;
 };
 
+// Sum over the waves of a workgroup of partial l (stride S per wave), as a
+// pairwise tree in a fixed order: three dependent adds for eight waves instead
+// of seven.
+template <int NW, int S, typename T>
+__device__ __forceinline__ T waves_total(const T *wsum, int l) {
+  if constexpr (NW == 1) {
+    return wsum[l];
+  } else {
+    return waves_total<NW / 2, S>(wsum, l) + waves_total<NW - NW / 2, S>(wsum + (NW / 2) * S, l);
+  }
+}
+
 // Status of one exchange within the item (all threads agree after a gather).
 // BC: row-broadcast layout (one lane per matrix row, 16 lanes per knot, every
 // lane holding its whole row of 3n columns).
@@ -147,9 +159,7 @@ struct RegItem {
     // lanes 0..2 of wave 0: scalar l summed over the waves in order, published
     const int l = threadIdx.x;
     if (l >= 3) return;
-    T tv = wsum[l];
-#pragma unroll
-    for (int i = 1; i < NWAVES; ++i) tv += wsum[i * 3 + l];
+    T tv = waves_total<NWAVES, 3>(wsum, l);
     if (G == 1) {
       tot[l] = tv;
       return;
@@ -351,9 +361,7 @@ struct RegItem {
     __syncthreads();
     const int l = threadIdx.x;
     if (l >= 6) return;
-    T tv = wsum[l];
-#pragma unroll
-    for (int i = 1; i < NWAVES; ++i) tv += wsum[i * 6 + l];
+    T tv = waves_total<NWAVES, 6>(wsum, l);
     if (G == 1) {
       tot[l] = tv;
       return;
