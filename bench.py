@@ -90,6 +90,8 @@ def main():
 
     # the autotuner picks the layout of this shape (timed on these inputs)
     tune = None if args.no_tune else gbdpcg.tune_single(S, g, n, N, P, **kw)
+    # latency floor of one exchange round at this layout (exchange alone, no work)
+    floor = gbdpcg.exchange_floor(tune["wgs"] if tune else 16, tune["threads"] if tune else 256, rounds=4000)
 
     # one zeroed lambda per step so every step is a full solve from lambda0 = 0
     K, W = args.steps, args.warmup
@@ -207,7 +209,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_kernel": traffic_kernel,
                 "algorithmic_bytes": hbm_bytes,
-                "note": "latency-bound: one cross-workgroup exchange per iteration; see per_iteration_us"}
+                "note": "latency-bound: one cross-workgroup exchange per iteration; per_iteration_us against "
+                        "exchange_floor_us (the exchange alone at the same layout)"}
 
     # batched forward + adjoint (64 systems, same shape), one persistent launch
     batched = None
@@ -292,6 +295,7 @@ def main():
                    "iters": iters, "rel_err_vs_fp64_oracle": err, "rel_err_vs_fp64_dense_cholesky": chol_err},
         "roofline": roofline, "cpu_baseline": cpu,
         "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
+        "exchange_floor_us": round(floor["us_per_round"], 3), "exchange_xcd_local": floor["xcd_local"],
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
         "reference_recurrence_us": round(ref_us, 2), "precond_ss_us": round(precond_us, 2),
         "n14_sizes": sweep, "tune": tune, "batched": batched, "workloads": workloads,

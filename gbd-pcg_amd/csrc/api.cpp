@@ -633,6 +633,73 @@ int gbdpcg_check_occupancy(int dtype, uint32_t n, uint32_t N, uint32_t *max_wgs)
   return GBDPCG_OK;
 }
 
+int gbdpcg_exchange_floor(uint32_t wgs, uint32_t threads, uint32_t rounds, double *us_per_round,
+                          uint32_t *xcd_local) {
+  if (wgs == 0 || wgs > (uint32_t)kMaxWgsPerSystem || (threads != 64 && threads != 256 && threads != 512) ||
+      rounds < 20 || rounds > 1000000 || !us_per_round) {
+    set_error("gbdpcg_exchange_floor: invalid argument");
+    return GBDPCG_ERR_INVALID;
+  }
+  DeviceInfo di;
+  int rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  int occ = 0;
+  HIPCHK(exchange_floor_occupancy((int)threads, &occ));
+  if (occ <= 0 || wgs > (uint32_t)di.cus * (uint32_t)occ) {
+    set_error("gbdpcg_exchange_floor: %u workgroups of %u threads are not co-resident", wgs, threads);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  // the solvers' placement: all ranks on one XCD when they fit there
+  const uint32_t stride = wgs <= (uint32_t)(di.cus / kXcds) * (uint32_t)occ ? (uint32_t)kXcds : 1u;
+  const uint32_t allow_xcd = (exchange_flags() & KF_NOXCD) ? 0u : 1u;
+  uint64_t *slots = nullptr;
+  int *out = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  auto cleanup = [&]() {
+    if (slots) (void)hipFree(slots);
+    if (out) (void)hipFree(out);
+    for (hipEvent_t &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  hipError_t e;
+  const uint32_t short_rounds = rounds / 10;
+  float t_short = 0.f, t_long = 0.f;
+  int h_out = 0;
+  if ((e = hipMalloc(&slots, 2 * (size_t)wgs * 4 * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMalloc(&out, sizeof(int))) != hipSuccess ||
+      (e = hipMemset(slots, 0, 2 * (size_t)wgs * 4 * sizeof(uint64_t))) != hipSuccess ||
+      (e = hipMemset(out, 0, sizeof(int))) != hipSuccess)
+    goto fail;
+  for (hipEvent_t &x : ev)
+    if ((e = hipEventCreate(&x)) != hipSuccess) goto fail;
+  // a warm-up launch, then a short and a long run: their difference is the
+  // per-round time without the launch and set-up costs
+  if ((e = exchange_floor_launch((int)threads, slots, wgs, short_rounds, stride, allow_xcd, out, nullptr)) !=
+          hipSuccess ||
+      (e = hipEventRecord(ev[0], nullptr)) != hipSuccess ||
+      (e = exchange_floor_launch((int)threads, slots, wgs, short_rounds, stride, allow_xcd, out, nullptr)) !=
+          hipSuccess ||
+      (e = hipEventRecord(ev[1], nullptr)) != hipSuccess || (e = hipEventRecord(ev[2], nullptr)) != hipSuccess ||
+      (e = exchange_floor_launch((int)threads, slots, wgs, rounds, stride, allow_xcd, out, nullptr)) != hipSuccess ||
+      (e = hipEventRecord(ev[3], nullptr)) != hipSuccess || (e = hipEventSynchronize(ev[3])) != hipSuccess ||
+      (e = hipEventElapsedTime(&t_short, ev[0], ev[1])) != hipSuccess ||
+      (e = hipEventElapsedTime(&t_long, ev[2], ev[3])) != hipSuccess ||
+      (e = hipMemcpy(&h_out, out, sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+    goto fail;
+  cleanup();
+  if (h_out & 1) {
+    set_error("gbdpcg_exchange_floor: a wait timed out (workgroups not co-resident?)");
+    return GBDPCG_ERR_TIMEOUT;
+  }
+  *us_per_round = (double)(t_long - t_short) * 1000.0 / (double)(rounds - short_rounds);
+  if (xcd_local) *xcd_local = (h_out & 2) ? 1u : 0u;
+  return GBDPCG_OK;
+fail:
+  set_error("gbdpcg_exchange_floor: %s", hipGetErrorString(e));
+  cleanup();
+  return GBDPCG_ERR_DEVICE;
+}
+
 int gbdpcg_solve_f32(const float *h_S, const float *h_Pinv, const float *h_gamma, float *h_lambda,
                      uint32_t stateSize, uint32_t knotPoints, const gbdpcg_config *cfg) {
   return solve_host_impl<float>(GBDPCG_F32, h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, cfg);

@@ -509,4 +509,93 @@ hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t coun
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Exchange floor: rounds of the solvers' exchange alone (every rank publishes
+// three words into one of two alternating records, wave 0 waits for every
+// rank's words, then a workgroup barrier) -- the latency under one PCG
+// iteration.  Round 0 checks the placement as the solvers do (all ranks on one
+// XCD: the later rounds' stores stay in its L2).  Workgroup b is rank b/stride.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ void __launch_bounds__(NT) exchange_floor_kernel(uint64_t *slots, uint32_t G, uint32_t rounds,
+                                                            uint32_t stride, uint32_t allow_xcd, int *out) {
+  __shared__ int ctl[4];
+  if (blockIdx.x % stride) return;
+  const uint32_t rank = blockIdx.x / stride;
+  if (rank >= G) return;
+  const int t = (int)threadIdx.x;
+  auto rec = [&](uint32_t type, uint32_t g) { return slots + ((size_t)type * G + g) * 4; };
+  if (t == 0) {
+    const uint32_t ea = (uint32_t)(xload(rec(0, rank)) >> 32);
+    const uint32_t eb = (uint32_t)(xload(rec(1, rank)) >> 32);
+    uint32_t base = (ea > eb ? ea : eb) + 1u;
+    if (base > 0xFFF00000u) base = 1u;
+    ctl[0] = 0;
+    ctl[1] = (int)base;
+  }
+  __syncthreads();
+  const uint32_t base = (uint32_t)ctl[1];
+  bool xcd = false;
+  for (uint32_t r = 0; r <= rounds; ++r) {
+    const uint32_t epoch = base + r;
+    const uint32_t type = r & 1u;
+    if (t < 3) {
+      const uint32_t x = 1u + (uint32_t)xcc_id();
+      const uint32_t v = r == 0 ? (t == 0 ? 1u : t == 1 ? x : x * x) : rank + (uint32_t)t;
+      xstore_sel(rec(type, rank) + t, v, epoch, xcd);
+    }
+    if (t < 64) {
+      float s1 = 0.f, s2 = 0.f;  // sums of small integers: exact in float
+      bool ok = true;
+      for (uint32_t g = (uint32_t)t; g < G; g += 64) {
+        uint32_t w[3];
+        if (!xwait_words<3>(rec(type, g), epoch, w)) {
+          ok = false;
+          break;
+        }
+        s1 += (float)w[1];
+        s2 += (float)w[2];
+      }
+      if (!ok) ctl[0] = 1;
+      if (r == 0) {
+        s1 = wave_sum(s1);
+        s2 = wave_sum(s2);
+        if (t == 0) ctl[2] = (allow_xcd && G > 1 && (double)s1 * s1 == (double)G * s2) ? 1 : 0;
+      }
+    }
+    __syncthreads();
+    if (__builtin_amdgcn_readfirstlane(ctl[0]) != 0) break;
+    if (r == 0) xcd = __builtin_amdgcn_readfirstlane(ctl[2]) != 0;
+  }
+  if (t == 0) {
+    if (ctl[0]) __hip_atomic_fetch_or(out, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (rank == 0 && xcd) __hip_atomic_fetch_or(out, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+namespace {
+const void *floor_kernel(int nt) {
+  switch (nt) {
+    case 64: return reinterpret_cast<const void *>(&exchange_floor_kernel<64>);
+    case 256: return reinterpret_cast<const void *>(&exchange_floor_kernel<256>);
+    case 512: return reinterpret_cast<const void *>(&exchange_floor_kernel<512>);
+    default: return nullptr;
+  }
+}
+}  // namespace
+
+hipError_t exchange_floor_occupancy(int nt, int *wgs_per_cu) {
+  const void *k = floor_kernel(nt);
+  if (!k) return hipErrorInvalidValue;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(wgs_per_cu, k, nt, 0);
+}
+
+hipError_t exchange_floor_launch(int nt, uint64_t *slots, uint32_t G, uint32_t rounds, uint32_t stride,
+                                 uint32_t allow_xcd, int *out, hipStream_t stream) {
+  const void *k = floor_kernel(nt);
+  if (!k) return hipErrorInvalidValue;
+  void *args[] = {&slots, &G, &rounds, &stride, &allow_xcd, &out};
+  return hipLaunchKernel(k, dim3(G * stride), dim3(nt), args, 0, stream);
+}
+
 }  // namespace gbdpcg

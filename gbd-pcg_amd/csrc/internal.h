@@ -143,6 +143,10 @@ hipError_t warm_start_launch(int dtype, uint32_t n, uint32_t N, void *lambda, vo
                              hipStream_t stream);
 hipError_t batch_warm_start_launch(int dtype, const KSys *systems, uint32_t count, uint64_t seed, double scale,
                                    hipStream_t stream);
+hipError_t exchange_floor_occupancy(int nt, int *wgs_per_cu);
+// out: bit 0 = a wait hit its bound, bit 1 = the rounds used XCD-local stores
+hipError_t exchange_floor_launch(int nt, uint64_t *slots, uint32_t G, uint32_t rounds, uint32_t stride,
+                                 uint32_t allow_xcd, int *out, hipStream_t stream);
 // pcg_reg_kernel.hip (register-resident latency kernel) and its variants.
 struct RegVariant {
   bool fused = true;  // fused recurrence; false: reference recurrence (256 threads only)

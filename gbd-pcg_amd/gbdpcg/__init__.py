@@ -126,6 +126,7 @@ SIGNATURES = [
     ("gbdpcg_lqr_schur_device", _i32, [_i32, _u32, _u32, _u32, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("gbdpcg_randn", _i32, [_i32, _u64, _u64, _dbl, _vp]),
     ("gbdpcg_counter_randn", _i32, [_i32, _u64, _u64, _u64, _dbl, _vp]),
+    ("gbdpcg_exchange_floor", _i32, [_u32, _u32, _u32, ctypes.POINTER(_dbl), ctypes.POINTER(_u32)]),
     ("gbdpcg_warm_start_device", _i32, [_i32, _u32, _u32, _vp, _vp, _vp, _vp]),
 ]
 
@@ -370,6 +371,14 @@ def tune_single(S, gamma, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, e
     check(lib().gbdpcg_tune_single(_dtype_code(S.dtype), n, N, _dev(S, "S"), _dev(Pinv, "Pinv"),
                                    _dev(gamma, "gamma"), ctypes.byref(cfg), int(trials), ctypes.byref(rep)))
     return rep.as_dict()
+
+
+def exchange_floor(wgs: int, threads: int = 256, rounds: int = 2000) -> dict:
+    """Time per round of the solvers' exchange alone (wgs workgroups placed as a
+    system's ranks): the latency under one PCG iteration."""
+    us, xcd = ctypes.c_double(), ctypes.c_uint32()
+    check(lib().gbdpcg_exchange_floor(int(wgs), int(threads), int(rounds), ctypes.byref(us), ctypes.byref(xcd)))
+    return {"us_per_round": us.value, "xcd_local": bool(xcd.value)}
 
 
 def lds_bytes(dtype, n: int, knots_per_wg: int, wgs: int) -> int:

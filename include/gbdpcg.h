@@ -130,6 +130,16 @@ const char *gbdpcg_version(void);
  * include/pcg.cuh:13-20, computed from the actual carve-up).  0 on bad input. */
 size_t gbdpcg_lds_bytes(int dtype, uint32_t n, uint32_t knots_per_wg, uint32_t wgs);
 
+/* Latency floor of one exchange round (extension, diagnostic): `wgs` workgroups
+ * of `threads` threads (64, 256 or 512) run `rounds` (>= 20) rounds of the
+ * solvers' exchange alone -- every rank publishes three words, waits for every
+ * rank's words, then a workgroup barrier -- placed as the solvers place a system
+ * (one XCD when it fits).  *us_per_round gets the time per round from the
+ * difference of a short and a long run; *xcd_local (may be NULL) whether the
+ * rounds used XCD-local stores.  What one PCG iteration cannot go below. */
+int gbdpcg_exchange_floor(uint32_t wgs, uint32_t threads, uint32_t rounds, double *us_per_round,
+                          uint32_t *xcd_local);
+
 /* Co-residency check (the reference's checkPcgOccupancy, include/pcg.cuh:23-49, which
  * exits the process; this returns a code).  On success *max_wgs receives the number
  * of workgroups per system that can be co-resident for this (n, N).  */

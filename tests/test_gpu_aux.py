@@ -252,6 +252,21 @@ def test_one_wave_ranks(gpu, dtype, n, N, recurrence):
     assert rel(lam.cpu().numpy(), lam_ref) <= (1e-4 if dtype == np.float32 else 1e-10)
 
 
+@pytest.mark.parametrize("wgs,threads", [(1, 256), (16, 256), (16, 512), (64, 64), (200, 256)])
+def test_exchange_floor(gpu, wgs, threads):
+    """The exchange-only rounds finish (every wait ends) and give a plausible
+    per-round latency; bad arguments are rejected."""
+    import gbdpcg
+    r = gbdpcg.exchange_floor(wgs, threads, rounds=500)
+    print(f"exchange floor: {wgs} x {threads}: {r['us_per_round']:.3f} us/round, xcd_local={r['xcd_local']}")
+    assert 0.0 < r["us_per_round"] < 20.0
+    assert not (wgs == 1 and r["xcd_local"])
+    for bad in (dict(wgs=0, threads=256), dict(wgs=4, threads=128), dict(wgs=4, threads=256, rounds=5)):
+        with pytest.raises(gbdpcg.GbdpcgError) as e:
+            gbdpcg.exchange_floor(**bad)
+        assert e.value.code == gbdpcg.ERR_INVALID
+
+
 def test_register_kernel_declines_unsupported(gpu):
     import gbdpcg
     n, N = 7, 16
