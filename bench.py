@@ -36,6 +36,11 @@ sys.path.insert(0, os.path.join(ROOT, "gbd-pcg_amd"))
 
 METRIC = "µs/solve to 1e-5 rel. resid (n=14,N=128); batched fwd+adjoint solves/sec"
 HBM_PEAK_GBS = 8000.0
+# LDS: 256 CUs x 256 B/clk (gfx950's LDS read rate; writes are half that) x 2.4 GHz
+LDS_PEAK_GBS = 256 * 256 * 2.4
+# FP32 VALU without packed-FP32 instructions (the build disables them): 256 CUs x 64
+# lanes x 2 flops (FMA) x 2.4 GHz
+VALU_F32_PEAK_TFLOPS = 256 * 64 * 2 * 2.4e9 / 1e12
 
 
 def main():
@@ -125,7 +130,8 @@ def main():
         gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw)
         ev[2 * i + 1].record(stream)
     torch.cuda.synchronize()
-    kern_us = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(kn)])) * 1000.0
+    kern_each = [ev[2 * i].elapsed_time(ev[2 * i + 1]) * 1000.0 for i in range(kn)]
+    kern_us = float(np.mean(kern_each))
 
     # end-to-end drop-in call (launch + wait + iteration count back to the host)
     lam_tmp = torch.zeros(n * N, device=dev)
@@ -251,6 +257,21 @@ def main():
         torch.cuda.synchronize()
         batch_us = t_start.elapsed_time(t_end) * 1000.0 / nb
         rate = B / (batch_us * 1e-6)
+        # the batch kernel's own duration: events on the plan's stream around each
+        # eager run (lambda0 zeroed before the first event)
+        bev = []
+        for _ in range(20):
+            with torch.cuda.stream(gs):
+                torch._foreach_zero_(lams64)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(gs)
+            plan.run(stream=gs)
+            b.record(gs)
+            bev.append((a, b))
+        torch.cuda.synchronize()
+        bkern_each = [a.elapsed_time(b) * 1000.0 for a, b in bev]
+        bres = plan.results()
+        broof = batched_roofline(np, gbdpcg, plan, bres, n, N, float(np.mean(bkern_each)))
         e_lam = e_gg = 0.0
         for d, (Sb64, _, gb64, gl64) in zip(systems, host64):
             e_lam = max(e_lam, rel_err(np, d["lam"].cpu().numpy(), banded_cholesky_solve(np, Sb64, gb64, n, N)))
@@ -261,6 +282,7 @@ def main():
                    "mean_iters_adj": float(np.mean([r.adj_iters for r in bres])),
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
                    "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg, "systems_checked": B},
+                   "kernel_us": spread(np, bkern_each), "roofline": broof,
                    "plan": plan.info(), "tune": plan.tune_report,
                    "cpu_baseline": cpu_batched(np, host64, n, N, tol, max_iter, args.cpu_seconds)}
 
@@ -294,13 +316,59 @@ def main():
                                "max_iter 173, lambda0=0", "n": n, "N": N, "recurrence": "fused (Chronopoulos-Gear)",
                    "iters": iters, "rel_err_vs_fp64_oracle": err, "rel_err_vs_fp64_dense_cholesky": chol_err},
         "roofline": roofline, "cpu_baseline": cpu,
-        "kernel_us": round(kern_us, 3), "per_iteration_us": round(kern_us / max(iters, 1), 3),
+        "kernel_us": round(kern_us, 3), "kernel_us_spread": spread(np, kern_each), "per_iteration_us": round(kern_us / max(iters, 1), 3),
         "exchange_floor_us": round(floor["us_per_round"], 3), "exchange_xcd_local": floor["xcd_local"],
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
         "reference_recurrence_us": round(ref_us, 2), "precond_ss_us": round(precond_us, 2),
         "n14_sizes": sweep, "tune": tune, "batched": batched, "workloads": workloads,
     }
     print(json.dumps(out), flush=True)
+
+
+def spread(np, xs):
+    """Mean and 10th / 50th / 90th percentiles of per-launch times (us)."""
+    p10, p50, p90 = np.percentile(np.asarray(xs, dtype=np.float64), [10, 50, 90])
+    return {"mean": round(float(np.mean(xs)), 3), "p10": round(float(p10), 3), "p50": round(float(p50), 3),
+            "p90": round(float(p90), 3), "launches": len(xs)}
+
+
+def batched_roofline(np, gbdpcg, plan, res, n, N, kern_us):
+    """Roofline of the batched forward + adjoint launch.  The systems' vectors and
+    blocks live in registers and LDS for the whole solve, so the bounds are the
+    LDS rate and the FP32 VALU rate, not HBM:
+      - LDS bytes per launch: rocprofv3 SQ_INSTS_LDS_LOAD_BANDWIDTH +
+        SQ_INSTS_LDS_STORE_BANDWIDTH (64-byte units) of this kernel, from the
+        committed counter summary, over the kernel time measured here;
+      - algorithmic flops: per iteration and system, two block-tridiagonal
+        products (S p and Pinv r: 2 x 2 x (3N - 2) n^2) and ten vector flops per
+        entry (2 dot products, 3 axpys) = 4 (3N - 2) n^2 + 10 nN, times the
+        forward plus adjoint iterations each system ran."""
+    t = plan.tune_report or {}
+    threads = t.get("threads") or 512
+    fam = "gbdpcg::pcg_pair_kernel<float, 14, "
+    name = f"{fam}{threads}, false>"
+    lds, lds_kernel = None, None
+    counters = os.path.join(ROOT, "profiles", "pcg_reg_kernel_n14_N128_counters.json")
+    if os.path.exists(counters):
+        with open(counters) as f:
+            kernels = json.load(f)["kernels"]
+        for k in [name] + sorted(k for k in kernels if k.startswith(fam)):
+            if k in kernels and kernels[k].get("lds_bytes"):
+                lds, lds_kernel = float(kernels[k]["lds_bytes"]), k
+                break
+    per_iter = 4 * (3 * N - 2) * n * n + 10 * n * N
+    flops = per_iter * sum(r.iters + r.adj_iters for r in res)
+    tflops = flops / (kern_us * 1e-6) / 1e12
+    out = {"bound": "lds", "achieved": None, "peak": round(LDS_PEAK_GBS, 1), "unit": "GB/s", "frac": None,
+           "lds_bytes": lds, "lds_kernel": lds_kernel, "algorithmic_flops": flops,
+           "valu_tflops": round(tflops, 3), "valu_peak_tflops": round(VALU_F32_PEAK_TFLOPS, 2),
+           "valu_frac": round(tflops / VALU_F32_PEAK_TFLOPS, 4),
+           "note": "latency-bound like the single solve: one exchange round per iteration; these fractions "
+                   "show how far the per-iteration work is from either resource"}
+    if lds is not None:
+        gbs = lds / (kern_us * 1e-6) / 1e9
+        out["achieved"], out["frac"] = round(gbs, 1), round(gbs / LDS_PEAK_GBS, 4)
+    return out
 
 
 def cpu_single(np, S_h, g_h, P_h, n, N, tol, max_iter, iters, lam_last, seconds):
@@ -312,13 +380,16 @@ def cpu_single(np, S_h, g_h, P_h, n, N, tol, max_iter, iters, lam_last, seconds)
     S64, g64, P64 = S_h.astype(np.float64), g_h.astype(np.float64), P_h.astype(np.float64)
     lam0 = np.zeros(n * N)
     O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
-    cnt, t0 = 0, time.perf_counter()
+    cnt, each, t0 = 0, [], time.perf_counter()
     while time.perf_counter() - t0 < seconds:
+        t1 = time.perf_counter()
         _, it_cpu, *_ = O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
+        each.append((time.perf_counter() - t1) * 1e6)
         cnt += 1
     cpu_us = (time.perf_counter() - t0) * 1e6 / cnt
     cpu = {"value": round(cpu_us, 2), "unit": "us/solve", "cores": 1, "kind": "port",
-           "sample": f"{cnt} FP64 solves of the same n=14,N=128 system (oracle/pcg_oracle.c, {it_cpu} iterations each)"}
+           "sample": f"{cnt} FP64 solves of the same n=14,N=128 system (oracle/pcg_oracle.c, {it_cpu} iterations each)",
+           "spread": spread(np, each)}
     lam_ref, *_ = O.pcg(S64, P64, g64, lam0, n, N, 0.0, iters)
     err = float(np.linalg.norm(lam_last - lam_ref) / np.linalg.norm(lam_ref))
     return cpu, err
