@@ -700,6 +700,81 @@ fail:
   return GBDPCG_ERR_DEVICE;
 }
 
+int gbdpcg_measure_rates(double *valu_f32_tflops, double *lds_read_gbs, double *hbm_read_gbs) {
+  DeviceInfo di;
+  int rc = device_info(0, &di);
+  if (rc < 0) return rc;
+  constexpr double kThreads = 256.0;
+  const uint32_t cus = (uint32_t)di.cus;
+  const uint32_t grid = cus * 16u;      // 16 workgroups of 256 threads per CU: 16 waves per CU
+  const size_t hbm_words = (size_t)1 << 27;  // 2 GiB of 16-byte words
+  void *buf = nullptr, *out = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  auto cleanup = [&]() {
+    if (buf) (void)hipFree(buf);
+    if (out) (void)hipFree(out);
+    for (hipEvent_t &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  };
+  hipError_t e;
+  // time of `reps` rounds with the launch and set-up costs removed: a short and
+  // a long run, after a warm-up
+  auto timed = [&](int which, uint32_t reps_short, uint32_t reps_long, const void *src, size_t count,
+                   double *ms) -> hipError_t {
+    hipError_t x;
+    float a = 0.f, b = 0.f;
+    if ((x = rate_launch(which, grid, reps_short, src, count, out, nullptr)) != hipSuccess ||
+        (x = hipEventRecord(ev[0], nullptr)) != hipSuccess ||
+        (x = rate_launch(which, grid, reps_short, src, count, out, nullptr)) != hipSuccess ||
+        (x = hipEventRecord(ev[1], nullptr)) != hipSuccess ||
+        (x = rate_launch(which, grid, reps_long, src, count, out, nullptr)) != hipSuccess ||
+        (x = hipEventRecord(ev[2], nullptr)) != hipSuccess || (x = hipEventSynchronize(ev[2])) != hipSuccess ||
+        (x = hipEventElapsedTime(&a, ev[0], ev[1])) != hipSuccess ||
+        (x = hipEventElapsedTime(&b, ev[1], ev[2])) != hipSuccess)
+      return x;
+    *ms = (double)b - (double)a;
+    return hipSuccess;
+  };
+  if ((e = hipMalloc(&out, (size_t)grid * sizeof(uint32_t))) != hipSuccess) goto fail;
+  for (hipEvent_t &x : ev)
+    if ((e = hipEventCreate(&x)) != hipSuccess) goto fail;
+  if (valu_f32_tflops) {
+    double ms = 0.0;
+    const uint32_t rs = 200, rl = 2200;  // 16 x 8 FMAs per lane per rep
+    if ((e = timed(0, rs, rl, nullptr, 0, &ms)) != hipSuccess) goto fail;
+    *valu_f32_tflops = ms > 0.0 ? (double)grid * kThreads * (rl - rs) * 16.0 * 8.0 * 2.0 / (ms * 1e-3) / 1e12 : 0.0;
+  }
+  if (lds_read_gbs) {
+    double ms = 0.0;
+    const uint32_t rs = 500, rl = 5500;  // 4 loads of 16 bytes per lane per rep
+    if ((e = timed(1, rs, rl, nullptr, 0, &ms)) != hipSuccess) goto fail;
+    *lds_read_gbs = ms > 0.0 ? (double)grid * kThreads * (rl - rs) * 4.0 * 16.0 / (ms * 1e-3) / 1e9 : 0.0;
+  }
+  if (hbm_read_gbs) {
+    double ms = 0.0;
+    float t = 0.f;
+    if ((e = hipMalloc(&buf, hbm_words * 16)) != hipSuccess || (e = hipMemset(buf, 1, hbm_words * 16)) != hipSuccess)
+      goto fail;
+    // a warm-up pass, then five timed passes
+    if ((e = rate_launch(2, grid, 0, buf, hbm_words, out, nullptr)) != hipSuccess ||
+        (e = hipEventRecord(ev[0], nullptr)) != hipSuccess)
+      goto fail;
+    for (int i = 0; i < 5; ++i)
+      if ((e = rate_launch(2, grid, 0, buf, hbm_words, out, nullptr)) != hipSuccess) goto fail;
+    if ((e = hipEventRecord(ev[1], nullptr)) != hipSuccess || (e = hipEventSynchronize(ev[1])) != hipSuccess ||
+        (e = hipEventElapsedTime(&t, ev[0], ev[1])) != hipSuccess)
+      goto fail;
+    ms = (double)t / 5.0;
+    *hbm_read_gbs = ms > 0.0 ? (double)hbm_words * 16.0 / (ms * 1e-3) / 1e9 : 0.0;
+  }
+  cleanup();
+  return GBDPCG_OK;
+fail:
+  set_error("gbdpcg_measure_rates: %s", hipGetErrorString(e));
+  cleanup();
+  return GBDPCG_ERR_DEVICE;
+}
+
 int gbdpcg_solve_f32(const float *h_S, const float *h_Pinv, const float *h_gamma, float *h_lambda,
                      uint32_t stateSize, uint32_t knotPoints, const gbdpcg_config *cfg) {
   return solve_host_impl<float>(GBDPCG_F32, h_S, h_Pinv, h_gamma, h_lambda, stateSize, knotPoints, cfg);

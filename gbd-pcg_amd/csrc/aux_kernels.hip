@@ -598,4 +598,94 @@ hipError_t exchange_floor_launch(int nt, uint64_t *slots, uint32_t G, uint32_t r
   return hipLaunchKernel(k, dim3(G * stride), dim3(nt), args, 0, stream);
 }
 
+// ---------------------------------------------------------------------------
+// Resource rates measured on streaming kernels, the peaks the solvers'
+// rooflines are read against.  Every access is in bounds by construction and
+// no kernel waits on another workgroup.
+// ---------------------------------------------------------------------------
+constexpr int kRateThreads = 256;
+
+// FP32 FMA: eight independent chains per lane (the build emits no packed-FP32
+// instructions, so each FMA is one lane-op).  The final store never happens for
+// the arguments used (the chains converge to c / (1 - m)); it keeps them live.
+__global__ void __launch_bounds__(kRateThreads) valu_rate_kernel(float m, float c, uint32_t reps, float *out) {
+  float a[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = (float)(threadIdx.x + i) * 1e-3f;
+  for (uint32_t r = 0; r < reps; ++r) {
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) a[i] = __builtin_fmaf(a[i], m, c);
+  }
+  const float t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  if (t == -1.0f) out[blockIdx.x] = t;
+}
+
+// LDS reads: 16-byte loads of consecutive lanes (conflict-free), four per step
+// from one of two alternating halves of a 32 KiB array; two three-input integer
+// adds per load keep the VALU well below its own rate.
+__global__ void __launch_bounds__(kRateThreads) lds_rate_kernel(uint32_t reps, uint32_t *out) {
+  constexpr int kEntries = 8 * kRateThreads;  // uint4 entries: 32 KiB
+  __shared__ uint4 buf[kEntries];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < (uint32_t)kEntries; i += kRateThreads) buf[i] = make_uint4(i, 3u * i, 5u * i, 7u * i);
+  __syncthreads();
+  uint32_t acc0 = 0, acc1 = 0, half = 0;
+  for (uint32_t r = 0; r < reps; ++r) {
+    // the empty asm makes the offset opaque, so the loads cannot be hoisted out
+    // of the loop (the two halves would otherwise be loop-invariant)
+    asm volatile("" : "+v"(half));
+    const uint32_t base = t + half;  // < 8 * kRateThreads with s < 4 below
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s) {
+      const uint4 v = buf[base + s * kRateThreads];
+      acc0 = acc0 + v.x + v.y;
+      acc1 = acc1 + v.z + v.w;
+    }
+    half ^= 4u * kRateThreads;
+  }
+  if ((acc0 ^ acc1) == 0x9E3779B9u) out[blockIdx.x] = acc0;
+}
+
+// HBM reads: a grid-stride pass over `count` 16-byte words, four loads in flight
+// per lane.
+__global__ void __launch_bounds__(kRateThreads) hbm_rate_kernel(const uint4 *__restrict__ src, size_t count,
+                                                                uint32_t *out) {
+  const size_t step = (size_t)gridDim.x * kRateThreads;
+  uint32_t acc0 = 0, acc1 = 0;
+  size_t i = (size_t)blockIdx.x * kRateThreads + threadIdx.x;
+  for (; i + 3 * step < count; i += 4 * step) {
+    const uint4 v0 = src[i], v1 = src[i + step], v2 = src[i + 2 * step], v3 = src[i + 3 * step];
+    acc0 = acc0 + v0.x + v0.y + v1.x + v1.y + v2.x + v2.y + v3.x + v3.y;
+    acc1 = acc1 + v0.z + v0.w + v1.z + v1.w + v2.z + v2.w + v3.z + v3.w;
+  }
+  for (; i < count; i += step) {
+    const uint4 v = src[i];
+    acc0 = acc0 + v.x + v.y;
+    acc1 = acc1 + v.z + v.w;
+  }
+  if ((acc0 ^ acc1) == 0x9E3779B9u) out[blockIdx.x] = acc0;
+}
+
+hipError_t rate_launch(int which, uint32_t grid, uint32_t reps, const void *src, size_t count, void *out,
+                       hipStream_t stream) {
+  switch (which) {
+    case 0:
+      hipLaunchKernelGGL(valu_rate_kernel, dim3(grid), dim3(kRateThreads), 0, stream, 0.5f, 0.25f, reps,
+                         static_cast<float *>(out));
+      break;
+    case 1:
+      hipLaunchKernelGGL(lds_rate_kernel, dim3(grid), dim3(kRateThreads), 0, stream, reps,
+                         static_cast<uint32_t *>(out));
+      break;
+    case 2:
+      hipLaunchKernelGGL(hbm_rate_kernel, dim3(grid), dim3(kRateThreads), 0, stream,
+                         static_cast<const uint4 *>(src), count, static_cast<uint32_t *>(out));
+      break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace gbdpcg

@@ -147,6 +147,10 @@ hipError_t exchange_floor_occupancy(int nt, int *wgs_per_cu);
 // out: bit 0 = a wait hit its bound, bit 1 = the rounds used XCD-local stores
 hipError_t exchange_floor_launch(int nt, uint64_t *slots, uint32_t G, uint32_t rounds, uint32_t stride,
                                  uint32_t allow_xcd, int *out, hipStream_t stream);
+// resource-rate kernels of 256 threads: which 0 = FP32 FMA (out: float[grid]),
+// 1 = LDS reads, 2 = HBM reads of `count` 16-byte words at src (out: uint32[grid])
+hipError_t rate_launch(int which, uint32_t grid, uint32_t reps, const void *src, size_t count, void *out,
+                       hipStream_t stream);
 // pcg_reg_kernel.hip (register-resident latency kernel) and its variants.
 struct RegVariant {
   bool fused = true;  // fused recurrence; false: reference recurrence (256 threads only)

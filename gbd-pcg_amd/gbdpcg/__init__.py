@@ -127,6 +127,7 @@ SIGNATURES = [
     ("gbdpcg_randn", _i32, [_i32, _u64, _u64, _dbl, _vp]),
     ("gbdpcg_counter_randn", _i32, [_i32, _u64, _u64, _u64, _dbl, _vp]),
     ("gbdpcg_exchange_floor", _i32, [_u32, _u32, _u32, ctypes.POINTER(_dbl), ctypes.POINTER(_u32)]),
+    ("gbdpcg_measure_rates", _i32, [ctypes.POINTER(_dbl)] * 3),
     ("gbdpcg_warm_start_device", _i32, [_i32, _u32, _u32, _vp, _vp, _vp, _vp]),
 ]
 
@@ -379,6 +380,15 @@ def exchange_floor(wgs: int, threads: int = 256, rounds: int = 2000) -> dict:
     us, xcd = ctypes.c_double(), ctypes.c_uint32()
     check(lib().gbdpcg_exchange_floor(int(wgs), int(threads), int(rounds), ctypes.byref(us), ctypes.byref(xcd)))
     return {"us_per_round": us.value, "xcd_local": bool(xcd.value)}
+
+
+def measure_rates() -> dict:
+    """Measured device rates (FP32 FMA TFLOP/s without packed-FP32 instructions,
+    LDS read GB/s, HBM read GB/s) from streaming kernels: the peaks the solvers'
+    rooflines are read against."""
+    v, l, h = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    check(lib().gbdpcg_measure_rates(ctypes.byref(v), ctypes.byref(l), ctypes.byref(h)))
+    return {"valu_f32_tflops": v.value, "lds_read_gbs": l.value, "hbm_read_gbs": h.value}
 
 
 def lds_bytes(dtype, n: int, knots_per_wg: int, wgs: int) -> int:

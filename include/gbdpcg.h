@@ -140,6 +140,15 @@ size_t gbdpcg_lds_bytes(int dtype, uint32_t n, uint32_t knots_per_wg, uint32_t w
 int gbdpcg_exchange_floor(uint32_t wgs, uint32_t threads, uint32_t rounds, double *us_per_round,
                           uint32_t *xcd_local);
 
+/* Measured resource rates (extension, diagnostic; no counterpart in the
+ * reference): the rates the solvers' rooflines are read against, each from a
+ * streaming kernel over the whole device -- *valu_f32_tflops: FP32 FMA
+ * (one lane-op per FMA, no packed-FP32 instructions); *lds_read_gbs: 16-byte
+ * conflict-free LDS reads; *hbm_read_gbs: one pass over a 2 GiB buffer.
+ * Any pointer may be NULL (that rate is not measured).  Allocates and frees
+ * its own buffers; synchronous. */
+int gbdpcg_measure_rates(double *valu_f32_tflops, double *lds_read_gbs, double *hbm_read_gbs);
+
 /* Co-residency check (the reference's checkPcgOccupancy, include/pcg.cuh:23-49, which
  * exits the process; this returns a code).  On success *max_wgs receives the number
  * of workgroups per system that can be co-resident for this (n, N).  */

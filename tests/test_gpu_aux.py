@@ -310,3 +310,15 @@ def test_invalid_arguments(gpu):
     assert not lam.cpu().numpy().any()
     r = gbdpcg.solve(dS, dg, lam, n, N, None, tol=1e-5, max_iter=100, exit_mode=gbdpcg.EXIT_REL_RESID)
     assert r.status == gbdpcg.CONVERGED
+
+
+def test_measure_rates(gpu):
+    """The resource-rate kernels finish and give rates in the device's range
+    (nominal: FP32 FMA 78.6 TFLOP/s without packed-FP32, LDS reads 79-157 TB/s,
+    HBM 8 TB/s)."""
+    import gbdpcg
+    r = gbdpcg.measure_rates()
+    print(f"rates: {r}")
+    assert 10.0 < r["valu_f32_tflops"] < 120.0
+    assert 5_000.0 < r["lds_read_gbs"] < 250_000.0
+    assert 1_000.0 < r["hbm_read_gbs"] < 10_000.0
