@@ -327,6 +327,24 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def host_cpu():
+    """The host's CPU model and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cpu_model": model, "nproc": usable}
+
+
 def spread(np, xs):
     """Mean and 10th / 50th / 90th percentiles of per-launch times (us)."""
     p10, p50, p90 = np.percentile(np.asarray(xs, dtype=np.float64), [10, 50, 90])
@@ -384,7 +402,8 @@ def cpu_single(np, S_h, g_h, P_h, n, N, tol, max_iter, iters, lam_last, seconds)
     from oracle import oracle as O
     S64, g64, P64 = S_h.astype(np.float64), g_h.astype(np.float64), P_h.astype(np.float64)
     lam0 = np.zeros(n * N)
-    O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
+    for _ in range(10):  # warm-up
+        O.pcg(S64, P64, g64, lam0, n, N, tol, max_iter, O.EXIT_REL)
     cnt, each, t0 = 0, [], time.perf_counter()
     while time.perf_counter() - t0 < seconds:
         t1 = time.perf_counter()
@@ -394,7 +413,7 @@ def cpu_single(np, S_h, g_h, P_h, n, N, tol, max_iter, iters, lam_last, seconds)
     cpu_us = (time.perf_counter() - t0) * 1e6 / cnt
     cpu = {"value": round(cpu_us, 2), "unit": "us/solve", "cores": 1, "kind": "port",
            "sample": f"{cnt} FP64 solves of the same n=14,N=128 system (oracle/pcg_oracle.c, {it_cpu} iterations each)",
-           "spread": spread(np, each)}
+           "spread": spread(np, each), "host": host_cpu()}
     lam_ref, *_ = O.pcg(S64, P64, g64, lam0, n, N, 0.0, iters)
     err = float(np.linalg.norm(lam_last - lam_ref) / np.linalg.norm(lam_ref))
     return cpu, err
@@ -461,6 +480,7 @@ def cpu_batched(np, host, n, N, tol, max_iter, seconds):
                 break
     el = time.perf_counter() - t0
     return {"value": round(done / el, 1), "unit": "fwd+adj solves/s", "cores": threads, "kind": "port",
+            "host": host_cpu(),
             "sample": f"{done // 2} forward + adjoint FP64 solve pairs of the 64 n=14,N=128 systems "
                       f"(oracle/pcg_oracle.c), {threads} threads, {el:.1f} s"}
 

@@ -71,6 +71,13 @@ struct DeviceInfo {
   bool ok = false;
 };
 
+// the calling thread's current device (0 when it cannot be read: the call that
+// follows then reports the HIP error itself)
+static int cur_dev() {
+  int d = 0;
+  return hipGetDevice(&d) == hipSuccess ? d : 0;
+}
+
 static int device_info(int dev, DeviceInfo *out) {
   static std::mutex mu;
   static std::map<int, DeviceInfo> cache;
@@ -223,7 +230,7 @@ static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 // workgroup), all workgroups on one XCD when they fit there.
 static int plan_single(int dtype, uint32_t n, uint32_t N, bool fused, uint32_t want_G, uint32_t kcap, Layout *L) {
   DeviceInfo di;
-  int rc = device_info(0, &di);
+  int rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   const uint32_t maxG = fused ? N / 2 : N;
   if (maxG == 0) {
@@ -271,7 +278,7 @@ static int plan_single_reg(int dtype, uint32_t n, uint32_t N, uint32_t want_G, u
   const RegVariant rv{fused, bc, pair && fused && n <= 14};
   if (!reg_supported((int)n, bc) || N < 4) return GBDPCG_ERR_UNSUPPORTED;
   DeviceInfo di;
-  int rc = device_info(0, &di);
+  int rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   Layout spread;  // first layout found whose workgroups span several XCDs
   bool have_spread = false;
@@ -334,7 +341,7 @@ static std::string single_key(int dev, int dtype, uint32_t n, uint32_t N, bool f
 static int layout_from(int dtype, uint32_t n, uint32_t N, bool reg, bool bc, int nt, uint32_t G,
                        uint32_t stride, Layout *L, bool fused, bool pair = false) {
   DeviceInfo di;
-  int rc = device_info(0, &di);
+  int rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   if (G == 0 || G > N || (stride != 1 && stride != (uint32_t)kXcds)) return GBDPCG_ERR_UNSUPPORTED;
   Layout t;
@@ -627,7 +634,7 @@ int gbdpcg_check_occupancy(int dtype, uint32_t n, uint32_t N, uint32_t *max_wgs)
   rc = occupancy(dtype, L.lds, &occ);
   if (rc < 0) return rc;
   DeviceInfo di;
-  rc = device_info(0, &di);
+  rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   if (max_wgs) *max_wgs = (uint32_t)(di.cus * occ);
   return GBDPCG_OK;
@@ -641,7 +648,7 @@ int gbdpcg_exchange_floor(uint32_t wgs, uint32_t threads, uint32_t rounds, doubl
     return GBDPCG_ERR_INVALID;
   }
   DeviceInfo di;
-  int rc = device_info(0, &di);
+  int rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   int occ = 0;
   HIPCHK(exchange_floor_occupancy((int)threads, &occ));
@@ -702,7 +709,7 @@ fail:
 
 int gbdpcg_measure_rates(double *valu_f32_tflops, double *lds_read_gbs, double *hbm_read_gbs) {
   DeviceInfo di;
-  int rc = device_info(0, &di);
+  int rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   constexpr double kThreads = 256.0;
   const uint32_t cus = (uint32_t)di.cus;
@@ -1037,7 +1044,7 @@ int gbdpcg_precond_device(int dtype, uint32_t n, uint32_t N, const void *d_S, vo
     return GBDPCG_ERR_INVALID;
   }
   DeviceInfo di;
-  int rc = device_info(0, &di);
+  int rc = device_info(cur_dev(), &di);
   if (rc < 0) return rc;
   if ((7 * (size_t)n * n + n) * tsize(dtype) > di.max_lds) {
     set_error("gbdpcg_precond_device: n=%u too large for on-chip inversion", n);
