@@ -217,6 +217,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_kernel": traffic_kernel,
                 "algorithmic_bytes": hbm_bytes, "hbm_read_measured": round(rates["hbm_read_gbs"], 1),
+                "latency": {"per_iteration_us": round(kern_us / max(iters, 1), 3),
+                            "exchange_floor_us": round(floor["us_per_round"], 3),
+                            "frac": round(floor["us_per_round"] / (kern_us / max(iters, 1)), 4)},
                 "note": "latency-bound: one cross-workgroup exchange per iteration; per_iteration_us against "
                         "exchange_floor_us (the exchange alone at the same layout)"}
 
