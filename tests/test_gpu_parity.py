@@ -146,6 +146,44 @@ def test_edge_cases(gpu):
     check_against_oracle(S2, P2, g2, n, 2, dtype, 1e-12, 50, 1, 1)
 
 
+@pytest.mark.parametrize("n,N,dtype", [(14, 4096, np.float32), (28, 1024, np.float64)])
+def test_long_horizons(gpu, n, N, dtype):
+    """Long horizons (hundreds of workgroups in one persistent launch) to the same
+    bar.  Pinv is formed on the device (checked against its definition in
+    test_gpu_aux.py) and the oracle runs on that same Pinv."""
+    torch = _torch()
+    import gbdpcg
+    S, g = gbdpcg.lqr_generate(n, n // 2, N, 31 + N, dtype=np.float64)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    dS = torch.from_numpy(S).to("cuda", tdt)
+    dP = torch.empty_like(dS)
+    gbdpcg.precond_device(dS, dP, n, N, gbdpcg.PRECOND_SS)
+    P = dP.cpu().numpy()
+    res, lam, _ = check_against_oracle(S.astype(dtype), P, g.astype(dtype), n, N, dtype, 1e-5, 500, 1, 1)
+    assert res.status == gbdpcg.CONVERGED
+    print(f"n={n} N={N} {dtype.__name__}: {res.iters} iterations")
+
+
+@pytest.mark.parametrize("n,N,dtype", [(14, 8192, np.float32), (28, 2048, np.float64)])
+def test_beyond_one_device_is_rejected(gpu, n, N, dtype):
+    """A single system whose matrices cannot stay on chip in one persistent
+    launch is rejected before any launch (TOO_LARGE), with lambda untouched."""
+    torch = _torch()
+    import gbdpcg
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    S, g = gbdpcg.lqr_generate(n, n // 2, N, 7, dtype=dtype)
+    dS = torch.from_numpy(S).to("cuda")
+    dP = torch.empty_like(dS)
+    gbdpcg.precond_device(dS, dP, n, N, gbdpcg.PRECOND_SS)
+    dg = torch.from_numpy(g).to("cuda")
+    lam = torch.full((n * N,), 3.0, device="cuda", dtype=tdt)
+    with pytest.raises(gbdpcg.GbdpcgError) as e:
+        gbdpcg.solve(dS, dg, lam, n, N, dP, tol=1e-5, max_iter=173, exit_mode=gbdpcg.EXIT_REL_RESID,
+                     recurrence=gbdpcg.RECUR_FUSED)
+    assert e.value.code == gbdpcg.ERR_TOO_LARGE
+    assert bool((lam == 3.0).all())
+
+
 def test_warm_start_and_repeat(gpu):
     """Repeated calls on one stream reuse the exchange records (epochs continue)."""
     n, N, dtype = 14, 64, np.float32
