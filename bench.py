@@ -135,6 +135,21 @@ def main():
     kern_each = [ev[2 * i].elapsed_time(ev[2 * i + 1]) * 1000.0 for i in range(kn)]
     kern_us = float(np.mean(kern_each))
 
+    # the load phase alone (max_iter = 0: S and Pinv staged from HBM into registers,
+    # r0 formed, one exchange, lambda written back), timed the same way
+    kw0 = dict(kw, max_iter=0)
+    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(2 * kn)]
+    torch.cuda.synchronize()
+    for i in range(kn):
+        ev0[2 * i].record(stream)
+        gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw0)
+        ev0[2 * i + 1].record(stream)
+    torch.cuda.synchronize()
+    load_us = float(np.median([ev0[2 * i].elapsed_time(ev0[2 * i + 1]) * 1000.0 for i in range(kn)]))
+    lams[:kn].zero_()
+    # one iteration: the kernel less its fixed part (the max_iter = 0 launch)
+    per_iter_us = (float(np.median(kern_each)) - load_us) / max(iters, 1)
+
     # end-to-end drop-in call (launch + wait + iteration count back to the host)
     lam_tmp = torch.zeros(n * N, device=dev)
     t0 = time.perf_counter()
@@ -217,11 +232,16 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_kernel": traffic_kernel,
                 "algorithmic_bytes": hbm_bytes, "hbm_read_measured": round(rates["hbm_read_gbs"], 1),
-                "latency": {"per_iteration_us": round(kern_us / max(iters, 1), 3),
+                "load_phase": {"us": round(load_us, 3), "achieved": round(hbm_bytes / (load_us * 1e-6) / 1e9, 3),
+                               "frac": round(hbm_bytes / (load_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 6),
+                               "note": "max_iter = 0: staging, r0, one exchange, lambda out; latency-bound at "
+                                       "this size (under 40 KB per workgroup)"},
+                "latency": {"per_iteration_us": round(per_iter_us, 3),
                             "exchange_floor_us": round(floor["us_per_round"], 3),
-                            "frac": round(floor["us_per_round"] / (kern_us / max(iters, 1)), 4)},
-                "note": "latency-bound: one cross-workgroup exchange per iteration; per_iteration_us against "
-                        "exchange_floor_us (the exchange alone at the same layout)"}
+                            "frac": round(floor["us_per_round"] / per_iter_us, 4)},
+                "note": "latency-bound: one cross-workgroup exchange per iteration; per_iteration_us (kernel "
+                        "less its max_iter = 0 launch, over the iterations) against exchange_floor_us (the "
+                        "exchange alone at the same layout)"}
 
     # batched forward + adjoint (64 systems, same shape), one persistent launch
     batched = None
@@ -276,6 +296,29 @@ def main():
         torch.cuda.synchronize()
         bkern_each = [a.elapsed_time(b) * 1000.0 for a, b in bev]
         bres = plan.results()
+        # the load and store phases alone: the same plan at max_iter = 0 (S and Pinv
+        # staged, r0 of both sides, one exchange, lambda, g_gamma and g_S written)
+        for d, w in zip(systems, plan.wgs()):
+            d["max_iter"], d["wgs"] = 0, w  # the same workgroups per system as the timed plan
+        plan0 = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
+                             threads=plan.tune_report["threads"] if plan.tune_report else 0)
+        plan0_info = plan0.info()
+        for d in systems:
+            d["max_iter"] = max_iter
+            del d["wgs"]
+        lev = []
+        for _ in range(20):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(gs)
+            plan0.run(stream=gs)
+            b.record(gs)
+            lev.append((a, b))
+        torch.cuda.synchronize()
+        bload_us = float(np.median([a.elapsed_time(b) * 1000.0 for a, b in lev]))
+        plan0.close()
+        # algorithmic HBM bytes of forward + adjoint per system: S, Pinv, gamma,
+        # lambda0, g_lambda read; lambda, g_gamma, g_S written
+        bbytes = B * (9 * n * n * N + 5 * n * N) * 4
         broof = batched_roofline(np, gbdpcg, plan, bres, n, N, float(np.median(bkern_each)), rates)
         e_lam = e_gg = 0.0
         for d, (Sb64, _, gb64, gl64) in zip(systems, host64):
@@ -288,6 +331,13 @@ def main():
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
                    "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg, "systems_checked": B},
                    "kernel_us": spread(np, bkern_each), "roofline": broof,
+                   "load_store_phase": {"us": round(bload_us, 2), "algorithmic_bytes": bbytes, "bound": "hbm",
+                                        "achieved": round(bbytes / (bload_us * 1e-6) / 1e9, 1), "unit": "GB/s",
+                                        "peak": HBM_PEAK_GBS,
+                                        "frac": round(bbytes / (bload_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                                        "frac_of_measured": round(bbytes / (bload_us * 1e-6) / 1e9
+                                                                  / rates["hbm_read_gbs"], 4),
+                                        "plan": plan0_info},
                    "plan": plan.info(), "tune": plan.tune_report,
                    "cpu_baseline": cpu_batched(np, host64, n, N, tol, max_iter, args.cpu_seconds)}
 
@@ -321,7 +371,8 @@ def main():
                                "max_iter 173, lambda0=0", "n": n, "N": N, "recurrence": "fused (Chronopoulos-Gear)",
                    "iters": iters, "rel_err_vs_fp64_oracle": err, "rel_err_vs_fp64_dense_cholesky": chol_err},
         "roofline": roofline, "cpu_baseline": cpu,
-        "kernel_us": round(kern_us, 3), "kernel_us_spread": spread(np, kern_each), "per_iteration_us": round(kern_us / max(iters, 1), 3),
+        "kernel_us": round(kern_us, 3), "kernel_us_spread": spread(np, kern_each),
+        "per_iteration_us": round(per_iter_us, 3),
         "exchange_floor_us": round(floor["us_per_round"], 3), "exchange_xcd_local": floor["xcd_local"],
         "sync_dropin_us": round(sync_us, 2), "host_pointer_dropin_us": round(host_us, 2),
         "reference_recurrence_us": round(ref_us, 2), "precond_ss_us": round(precond_us, 2),

@@ -594,7 +594,10 @@ __device__ __forceinline__ T row_mv(const RegItem<T, NS, NT, BC> &it, const T (&
   }
 }
 
-template <typename T, int NS, int NT, bool FUSED, bool BC = false>
+// PRE: the set-up's HBM reads (initial guess window, first right-hand side)
+// issued together with the matrix chunks, one round trip instead of three
+// (off inside the paired kernel, whose 128-VGPR budget spills more with it).
+template <typename T, int NS, int NT, bool FUSED, bool BC = false, bool PRE = true>
 __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   using R = RegItem<T, NS, NT, BC>;
   constexpr int n = R::n, nP = R::nP, CW = R::CW;
@@ -659,6 +662,32 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   }
   for (int i = opaque_tid(); i < total; i += NT) lds[i] = (T)0;
 
+  const bool fwd = (d.flags & KF_FWD) != 0;
+  const bool adj = (d.flags & KF_ADJ) != 0;
+  // every HBM read of the set-up in one round trip (PRE): the initial guess
+  // window (own + one ghost knot each side; (K + 2) n <= NT values, one per
+  // thread below NT, the rest after the barrier) and the first right-hand side,
+  // issued with the matrix chunks
+  const T *lam0 = static_cast<const T *>(d.lambda);
+  const int wl = (K + 2) * n;
+  T lam_pre = 0;
+  int lam_pos = -1;
+  if constexpr (PRE) {
+    const int i = opaque_tid();
+    if (i < wl) {
+      const int jj = i / n, e = i - jj * n;
+      const int k = it.a - 1 + jj;
+      if (k >= 0 && k < it.N) {
+        lam_pre = gptr(lam0)[(size_t)k * n + e];
+        lam_pos = (jj + 1) * nP + e;
+      }
+    }
+  }
+  T b_pre = 0;
+  if constexpr (PRE) {
+    if (it.owner) b_pre = gptr(static_cast<const T *>(fwd ? d.gamma : d.glambda))[(size_t)kg * n + row];
+  }
+
   // matrix chunks straight from HBM into registers
   T mM[CW], mS[CW];
   if constexpr (BC) {
@@ -671,14 +700,10 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
   it.seq = 0;
-
-  const bool fwd = (d.flags & KF_FWD) != 0;
-  const bool adj = (d.flags & KF_ADJ) != 0;
-  // initial guess window (own + one ghost knot each side)
   {
-    const T *lam0 = static_cast<const T *>(d.lambda);
     T *dst = fwd ? it.xv : it.lv;
-    for (int i = opaque_tid(); i < (K + 2) * n; i += NT) {
+    if (lam_pos >= 0) dst[lam_pos] = lam_pre;
+    for (int i = opaque_tid() + (PRE ? NT : 0); i < wl; i += NT) {
       const int jj = i / n, e = i - jj * n;
       const int k = it.a - 1 + jj;
       if (k >= 0 && k < it.N) dst[(jj + 1) * nP + e] = gptr(lam0)[(size_t)k * n + e];
@@ -699,11 +724,12 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       for (int i = opaque_tid(); i < it.VL; i += NT) it.xv[i] = (T)0;
       __syncthreads();
     }
+    // with PRE the first pass's right-hand side was read with the set-up
     const T *rhs = static_cast<const T *>(pass == 0 ? d.gamma : d.glambda);
     T x_i = 0, b_i = 0;
     if (it.owner) {
       x_i = it.xv[own_pos];
-      b_i = gptr(rhs)[(size_t)kg * n + row];
+      b_i = PRE && (pass == 0) == fwd ? b_pre : gptr(rhs)[(size_t)kg * n + row];
     }
     // r = rhs - S x (own rows)
     T r_i;
@@ -1422,7 +1448,7 @@ __device__ __forceinline__ void pair_item(const KSys &d, uint32_t rank, T *lds) 
   if ((d.flags & (KF_FWD | KF_ADJ)) == (KF_FWD | KF_ADJ) && (d.flags & KF_FUSED))
     reg_item_pair<T, NS, NT, BC>(d, rank, lds);
   else
-    reg_item<T, NS, NT, true, BC>(d, rank, lds);
+    reg_item<T, NS, NT, true, BC, false>(d, rank, lds);
 }
 
 template <typename T, int NS, int NT, bool BC>
