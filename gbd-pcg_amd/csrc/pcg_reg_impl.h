@@ -475,7 +475,11 @@ __device__ __forceinline__ T group_sum(T v) {
 
 // Loads this lane's chunk of row `row` of block row kg of a cbtd matrix
 // (padded row-major columns [c*CW, (c+1)*CW) of the n x 3nP block row).
-template <typename T, int NS, int CW>
+// GUARDED: one guarded load per element (an exec-mask branch each, so the loads
+// issue slowly); kept where the branch-free form's live predicate masks cost
+// more than they save (the paired kernel's 128-VGPR allocation, the runtime-n
+// and n = 28 batch instances).
+template <typename T, int NS, int CW, bool GUARDED = false>
 __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A, int kg, int row, int c, int N,
                                            bool active) {
   constexpr int n = NS, nP = (int)padded_n(NS);
@@ -483,32 +487,64 @@ __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A
   // batched item loop (they would stay live, in registers, for the whole solve)
   int cl = c, rl = row;
   asm volatile("" : "+v"(cl), "+v"(rl));
+  if constexpr (GUARDED) {
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+      const int col = cl * CW + j;
+      const int seg = col / nP, cc = col - seg * nP;
+      T v = 0;
+      if (active && kg >= 0 && kg < N && seg < 3 && cc < n && !(kg == 0 && seg == 0) && !(kg == N - 1 && seg == 2))
+        v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)(seg * n + cc) * n + rl];
+      dst[j] = v;
+    }
+    return;
+  }
+  // branch-free: every lane loads (an absent element from offset 0, which is
+  // always in bounds) and a select zeroes what it must not hold, so the loads
+  // issue back to back instead of one exec-mask branch each; 32-bit offsets
+  // (3 n^2 N < 2^31 for every size the planner accepts)
+  const bool rowok = active & (kg >= 0) & (kg < N);
+  const uint32_t base = rowok ? (uint32_t)kg * (uint32_t)(3 * n * n) + (uint32_t)rl : 0u;
+  const bool first = kg == 0, last = kg == N - 1;
 #pragma unroll
   for (int j = 0; j < CW; ++j) {
     const int col = cl * CW + j;
     const int seg = col / nP, cc = col - seg * nP;
-    T v = 0;
-    if (active && kg >= 0 && kg < N && seg < 3 && cc < n && !(kg == 0 && seg == 0) && !(kg == N - 1 && seg == 2))
-      v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)(seg * n + cc) * n + rl];
-    dst[j] = v;
+    const bool ok = rowok & (seg < 3) & (cc < n) & !(first & (seg == 0)) & !(last & (seg == 2));
+    const T v = gptr(A)[ok ? base + (uint32_t)((seg * n + cc) * n) : 0u];
+    dst[j] = ok ? v : (T)0;
   }
 }
 
 // Row-broadcast layout: the whole row `row` of block row kg (3n values, block
 // column s, column k at index s*n + k).
-template <typename T, int NS>
+template <typename T, int NS, bool GUARDED = false>
 __device__ __forceinline__ void load_row(T (&dst)[3 * NS], const T *__restrict__ A, int kg, int row, int N,
                                          bool active) {
   constexpr int n = NS;
   int rl = row;
   asm volatile("" : "+v"(rl));
+  if constexpr (GUARDED) {
+#pragma unroll
+    for (int j = 0; j < 3 * n; ++j) {
+      const int seg = j / n;
+      T v = 0;
+      if (active && kg >= 0 && kg < N && !(kg == 0 && seg == 0) && !(kg == N - 1 && seg == 2))
+        v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)j * n + rl];
+      dst[j] = v;
+    }
+    return;
+  }
+  // branch-free, as load_chunk
+  const bool rowok = active & (kg >= 0) & (kg < N);
+  const uint32_t base = rowok ? (uint32_t)kg * (uint32_t)(3 * n * n) + (uint32_t)rl : 0u;
+  const bool first = kg == 0, last = kg == N - 1;
 #pragma unroll
   for (int j = 0; j < 3 * n; ++j) {
     const int seg = j / n;
-    T v = 0;
-    if (active && kg >= 0 && kg < N && !(kg == 0 && seg == 0) && !(kg == N - 1 && seg == 2))
-      v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)j * n + rl];
-    dst[j] = v;
+    const bool ok = rowok & !(first & (seg == 0)) & !(last & (seg == 2));
+    const T v = gptr(A)[ok ? base + (uint32_t)(j * n) : 0u];
+    dst[j] = ok ? v : (T)0;
   }
 }
 
@@ -595,9 +631,12 @@ __device__ __forceinline__ T row_mv(const RegItem<T, NS, NT, BC> &it, const T (&
 }
 
 // PRE: the set-up's HBM reads (initial guess window, first right-hand side)
-// issued together with the matrix chunks, one round trip instead of three
-// (off inside the paired kernel, whose 128-VGPR budget spills more with it).
-template <typename T, int NS, int NT, bool FUSED, bool BC = false, bool PRE = true>
+// issued together with the matrix chunks, one round trip instead of three (off
+// inside the paired kernel, whose 128-VGPR allocation spills differently, and
+// runs its batches slower, with it).
+// BF: the matrix chunks loaded branch-free (load_chunk), for the fixed-size
+// instances; the runtime-n instances keep the guarded loads.
+template <typename T, int NS, int NT, bool FUSED, bool BC = false, bool PRE = true, bool BF = PRE>
 __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   using R = RegItem<T, NS, NT, BC>;
   constexpr int n = R::n, nP = R::nP, CW = R::CW;
@@ -690,12 +729,16 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
 
   // matrix chunks straight from HBM into registers
   T mM[CW], mS[CW];
+  // branch-free loads for the fixed-size instances of n <= 14 (the single-solve
+  // layouts); the runtime-n and n = 28 instances keep the guarded form, whose
+  // fewer live masks leave their batches faster
+  constexpr bool kGuarded = !(BF && NS <= 14);
   if constexpr (BC) {
-    load_row<T, NS>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
-    load_row<T, NS>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
+    load_row<T, NS, kGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
+    load_row<T, NS, kGuarded>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
   } else {
-    load_chunk<T, NS, CW>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
-    load_chunk<T, NS, CW>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+    load_chunk<T, NS, CW, kGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW, kGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
   }
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
@@ -1097,11 +1140,11 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
 
   T mM[CW], mS[CW];
   if constexpr (BC) {
-    load_row<T, NS>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
-    load_row<T, NS>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
+    load_row<T, NS, true>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
+    load_row<T, NS, true>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
   } else {
-    load_chunk<T, NS, CW>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
-    load_chunk<T, NS, CW>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+    load_chunk<T, NS, CW, true>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW, true>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
   }
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
@@ -1373,18 +1416,18 @@ __device__ __forceinline__ void reg_dispatch(const KSys &d, uint32_t rank, T *ld
     reg_item<T, NS, NT, FUSED, BC>(d, rank, lds);
   } else if constexpr (BC) {
     switch (d.n) {
-      case 12: reg_item<T, 12, NT, FUSED, true>(d, rank, lds); break;
-      case 14: reg_item<T, 14, NT, FUSED, true>(d, rank, lds); break;
+      case 12: reg_item<T, 12, NT, FUSED, true, true, false>(d, rank, lds); break;
+      case 14: reg_item<T, 14, NT, FUSED, true, true, false>(d, rank, lds); break;
       default: break;  // the planner only admits the sizes above
     }
   } else {
     switch (d.n) {
-      case 2: reg_item<T, 2, NT, FUSED>(d, rank, lds); break;
-      case 6: reg_item<T, 6, NT, FUSED>(d, rank, lds); break;
-      case 12: reg_item<T, 12, NT, FUSED>(d, rank, lds); break;
-      case 14: reg_item<T, 14, NT, FUSED>(d, rank, lds); break;
+      case 2: reg_item<T, 2, NT, FUSED, false, true, false>(d, rank, lds); break;
+      case 6: reg_item<T, 6, NT, FUSED, false, true, false>(d, rank, lds); break;
+      case 12: reg_item<T, 12, NT, FUSED, false, true, false>(d, rank, lds); break;
+      case 14: reg_item<T, 14, NT, FUSED, false, true, false>(d, rank, lds); break;
       case 28:
-        if constexpr (NS == 0) reg_item<T, 28, NT, FUSED>(d, rank, lds);
+        if constexpr (NS == 0) reg_item<T, 28, NT, FUSED, false, true, false>(d, rank, lds);
         break;
       default: break;  // the planner only admits the sizes above
     }
