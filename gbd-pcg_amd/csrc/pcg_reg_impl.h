@@ -475,11 +475,44 @@ __device__ __forceinline__ T group_sum(T v) {
 
 // Loads this lane's chunk of row `row` of block row kg of a cbtd matrix
 // (padded row-major columns [c*CW, (c+1)*CW) of the n x 3nP block row).
-// GUARDED: one guarded load per element (an exec-mask branch each, so the loads
-// issue slowly); kept where the branch-free form's live predicate masks cost
-// more than they save (the paired kernel's 128-VGPR allocation, the runtime-n
-// and n = 28 batch instances).
-template <typename T, int NS, int CW, bool GUARDED = false>
+constexpr int kLoadTables = 0, kLoadArith = 1, kLoadGuarded = 2;
+
+// Per-chunk column tables of a block row (lane chunk c of CW padded columns):
+// the element offset of each column (0 for a padded column) and bit masks of
+// the valid columns, of those in the first block (S_{k,k-1}) and in the last
+// (S_{k,k+1}).
+template <int NS, int CW>
+struct ChunkTables {
+  uint32_t off[2][CW];
+  uint32_t valid[2], seg0[2], seg2[2];
+};
+template <int NS, int CW>
+__host__ __device__ constexpr ChunkTables<NS, CW> chunk_tables() {
+  constexpr int n = NS, nP = (int)padded_n(NS);
+  ChunkTables<NS, CW> t{};
+  for (int c = 0; c < 2; ++c) {
+    t.valid[c] = t.seg0[c] = t.seg2[c] = 0u;
+    for (int j = 0; j < CW; ++j) {
+      const int col = c * CW + j, seg = col / nP, cc = col - seg * nP;
+      const bool ok = seg < 3 && cc < n;
+      t.off[c][j] = ok ? (uint32_t)((seg * n + cc) * n) : 0u;
+      if (ok) t.valid[c] |= 1u << j;
+      if (ok && seg == 0) t.seg0[c] |= 1u << j;
+      if (ok && seg == 2) t.seg2[c] |= 1u << j;
+    }
+  }
+  return t;
+}
+
+// Matrix chunk loads.  kLoadTables: branch-free, offsets from compile-time
+// tables (fewest instructions, but the table constants take VGPRs: FP32
+// 256-thread fixed-size instances, the single-solve layout); kLoadArith:
+// branch-free, offsets computed per element (other fixed sizes n <= 14);
+// kLoadGuarded: one guarded load per element (an exec-mask branch each, so the
+// loads issue slowly), kept where the branch-free forms' live masks cost more
+// than they save (the paired kernel's 128-VGPR allocation, the runtime-n and
+// n = 28 batch instances).
+template <typename T, int NS, int CW, int MODE = kLoadTables>
 __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A, int kg, int row, int c, int N,
                                            bool active) {
   constexpr int n = NS, nP = (int)padded_n(NS);
@@ -487,7 +520,7 @@ __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A
   // batched item loop (they would stay live, in registers, for the whole solve)
   int cl = c, rl = row;
   asm volatile("" : "+v"(cl), "+v"(rl));
-  if constexpr (GUARDED) {
+  if constexpr (MODE == kLoadGuarded) {
 #pragma unroll
     for (int j = 0; j < CW; ++j) {
       const int col = cl * CW + j;
@@ -497,22 +530,44 @@ __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A
         v = gptr(A)[(size_t)kg * 3 * n * n + (size_t)(seg * n + cc) * n + rl];
       dst[j] = v;
     }
-    return;
-  }
-  // branch-free: every lane loads (an absent element from offset 0, which is
-  // always in bounds) and a select zeroes what it must not hold, so the loads
-  // issue back to back instead of one exec-mask branch each; 32-bit offsets
-  // (3 n^2 N < 2^31 for every size the planner accepts)
-  const bool rowok = active & (kg >= 0) & (kg < N);
-  const uint32_t base = rowok ? (uint32_t)kg * (uint32_t)(3 * n * n) + (uint32_t)rl : 0u;
-  const bool first = kg == 0, last = kg == N - 1;
+  } else if constexpr (MODE == kLoadArith) {
+    // branch-free, offsets computed per element (no table registers)
+    const bool rowok = active & (kg >= 0) & (kg < N);
+    const uint32_t base = rowok ? (uint32_t)kg * (uint32_t)(3 * n * n) + (uint32_t)rl : 0u;
+    const bool first = kg == 0, last = kg == N - 1;
 #pragma unroll
-  for (int j = 0; j < CW; ++j) {
-    const int col = cl * CW + j;
-    const int seg = col / nP, cc = col - seg * nP;
-    const bool ok = rowok & (seg < 3) & (cc < n) & !(first & (seg == 0)) & !(last & (seg == 2));
-    const T v = gptr(A)[ok ? base + (uint32_t)((seg * n + cc) * n) : 0u];
-    dst[j] = ok ? v : (T)0;
+    for (int j = 0; j < CW; ++j) {
+      const int col = cl * CW + j;
+      const int seg = col / nP, cc = col - seg * nP;
+      const bool ok = rowok & (seg < 3) & (cc < n) & !(first & (seg == 0)) & !(last & (seg == 2));
+      const T v = gptr(A)[ok ? base + (uint32_t)((seg * n + cc) * n) : 0u];
+      dst[j] = ok ? v : (T)0;
+    }
+  } else {
+    // branch-free: every lane loads (an absent element from an in-bounds offset:
+    // row rl of column 0 of its block row, or offset 0 for a row outside the
+    // system) and a select zeroes what it must not hold, so the loads issue back
+    // to back instead of one exec-mask branch each.  Column offsets and the masks
+    // of valid / first-block / last-block columns come from compile-time tables of
+    // the (at most two) chunks; 32-bit offsets (3 n^2 N < 2^31 for every size the
+    // planner accepts).
+    static_assert(3 * nP <= 2 * CW && CW <= 32, "branch-free loads: two chunks of at most 32 columns");
+    constexpr ChunkTables<NS, CW> tab = chunk_tables<NS, CW>();
+    const bool rowok = active & (kg >= 0) & (kg < N);
+    // byte offsets in 32 bits: one VGPR offset per load against the uniform base
+    const uint32_t base = rowok ? ((uint32_t)kg * (uint32_t)(3 * n * n) + (uint32_t)rl) * (uint32_t)sizeof(T) : 0u;
+    const bool c1 = cl != 0;
+    uint32_t mask = c1 ? tab.valid[1] : tab.valid[0];
+    if (kg == 0) mask &= ~(c1 ? tab.seg0[1] : tab.seg0[0]);
+    if (kg == N - 1) mask &= ~(c1 ? tab.seg2[1] : tab.seg2[0]);
+    mask = rowok ? mask : 0u;
+    const auto *bytes = reinterpret_cast<const __attribute__((address_space(1))) unsigned char *>(gptr(A));
+#pragma unroll
+    for (int j = 0; j < CW; ++j) {
+      const uint32_t off = base + (c1 ? tab.off[1][j] : tab.off[0][j]) * (uint32_t)sizeof(T);
+      const T v = *reinterpret_cast<const __attribute__((address_space(1))) T *>(bytes + off);
+      dst[j] = ((mask >> j) & 1u) ? v : (T)0;
+    }
   }
 }
 
@@ -733,12 +788,13 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   // layouts); the runtime-n and n = 28 instances keep the guarded form, whose
   // fewer live masks leave their batches faster
   constexpr bool kGuarded = !(BF && NS <= 14);
+  constexpr int kMode = kGuarded ? kLoadGuarded : (sizeof(T) == 4 && NT == 256) ? kLoadTables : kLoadArith;
   if constexpr (BC) {
     load_row<T, NS, kGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
     load_row<T, NS, kGuarded>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
   } else {
-    load_chunk<T, NS, CW, kGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
-    load_chunk<T, NS, CW, kGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+    load_chunk<T, NS, CW, kMode>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW, kMode>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
   }
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
@@ -1143,8 +1199,8 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
     load_row<T, NS, true>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
     load_row<T, NS, true>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
   } else {
-    load_chunk<T, NS, CW, true>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
-    load_chunk<T, NS, CW, true>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+    load_chunk<T, NS, CW, kLoadGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW, kLoadGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
   }
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
