@@ -54,6 +54,9 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="time the planner's default layout, not the tuned one")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the other single-solve sizes (N = 32, 64, 256), e.g. for per-kernel profiles")
+    ap.add_argument("--no-load-phase", action="store_true",
+                    help="skip the max_iter = 0 launches (set-up and load/store phases alone), e.g. so that a "
+                         "kernel trace holds full solves only")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -137,18 +140,20 @@ def main():
 
     # the load phase alone (max_iter = 0: S and Pinv staged from HBM into registers,
     # r0 formed, one exchange, lambda written back), timed the same way
-    kw0 = dict(kw, max_iter=0)
-    ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(2 * kn)]
-    torch.cuda.synchronize()
-    for i in range(kn):
-        ev0[2 * i].record(stream)
-        gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw0)
-        ev0[2 * i + 1].record(stream)
-    torch.cuda.synchronize()
-    load_us = float(np.median([ev0[2 * i].elapsed_time(ev0[2 * i + 1]) * 1000.0 for i in range(kn)]))
-    lams[:kn].zero_()
+    load_us = None
+    if not args.no_load_phase:
+        kw0 = dict(kw, max_iter=0)
+        ev0 = [torch.cuda.Event(enable_timing=True) for _ in range(2 * kn)]
+        torch.cuda.synchronize()
+        for i in range(kn):
+            ev0[2 * i].record(stream)
+            gbdpcg.solve_async(S, g, lams[i], n, N, P, dres, **kw0)
+            ev0[2 * i + 1].record(stream)
+        torch.cuda.synchronize()
+        load_us = float(np.median([ev0[2 * i].elapsed_time(ev0[2 * i + 1]) * 1000.0 for i in range(kn)]))
+        lams[:kn].zero_()
     # one iteration: the kernel less its fixed part (the max_iter = 0 launch)
-    per_iter_us = (float(np.median(kern_each)) - load_us) / max(iters, 1)
+    per_iter_us = (float(np.median(kern_each)) - (load_us or 0.0)) / max(iters, 1)
 
     # end-to-end drop-in call (launch + wait + iteration count back to the host)
     lam_tmp = torch.zeros(n * N, device=dev)
@@ -232,10 +237,11 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic, "traffic_kernel": traffic_kernel,
                 "algorithmic_bytes": hbm_bytes, "hbm_read_measured": round(rates["hbm_read_gbs"], 1),
-                "load_phase": {"us": round(load_us, 3), "achieved": round(hbm_bytes / (load_us * 1e-6) / 1e9, 3),
-                               "frac": round(hbm_bytes / (load_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 6),
-                               "note": "max_iter = 0: staging, r0, one exchange, lambda out; latency-bound at "
-                                       "this size (under 40 KB per workgroup)"},
+                "load_phase": None if load_us is None else {
+                    "us": round(load_us, 3), "achieved": round(hbm_bytes / (load_us * 1e-6) / 1e9, 3),
+                    "frac": round(hbm_bytes / (load_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 6),
+                    "note": "max_iter = 0: matrices to registers, r0, two exchanges, lambda out; latency-bound at "
+                            "this size (under 40 KB per workgroup)"},
                 "latency": {"per_iteration_us": round(per_iter_us, 3),
                             "exchange_floor_us": round(floor["us_per_round"], 3),
                             "frac": round(floor["us_per_round"] / per_iter_us, 4)},
@@ -298,27 +304,33 @@ def main():
         bres = plan.results()
         # the load and store phases alone: the same plan at max_iter = 0 (S and Pinv
         # staged, r0 of both sides, one exchange, lambda, g_gamma and g_S written)
-        for d, w in zip(systems, plan.wgs()):
-            d["max_iter"], d["wgs"] = 0, w  # the same workgroups per system as the timed plan
-        plan0 = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
-                             threads=plan.tune_report["threads"] if plan.tune_report else 0)
-        plan0_info = plan0.info()
-        for d in systems:
-            d["max_iter"] = max_iter
-            del d["wgs"]
-        lev = []
-        for _ in range(20):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(gs)
-            plan0.run(stream=gs)
-            b.record(gs)
-            lev.append((a, b))
-        torch.cuda.synchronize()
-        bload_us = float(np.median([a.elapsed_time(b) * 1000.0 for a, b in lev]))
-        plan0.close()
-        # algorithmic HBM bytes of forward + adjoint per system: S, Pinv, gamma,
-        # lambda0, g_lambda read; lambda, g_gamma, g_S written
-        bbytes = B * (9 * n * n * N + 5 * n * N) * 4
+        lsp = None
+        if not args.no_load_phase:
+            for d, w in zip(systems, plan.wgs()):
+                d["max_iter"], d["wgs"] = 0, w  # the same workgroups per system as the timed plan
+            plan0 = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED,
+                                 threads=plan.tune_report["threads"] if plan.tune_report else 0)
+            plan0_info = plan0.info()
+            for d in systems:
+                d["max_iter"] = max_iter
+                del d["wgs"]
+            lev = []
+            for _ in range(20):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(gs)
+                plan0.run(stream=gs)
+                b.record(gs)
+                lev.append((a, b))
+            torch.cuda.synchronize()
+            bload_us = float(np.median([a.elapsed_time(b) * 1000.0 for a, b in lev]))
+            plan0.close()
+            # algorithmic HBM bytes of forward + adjoint per system: S, Pinv, gamma,
+            # lambda0, g_lambda read; lambda, g_gamma, g_S written
+            bbytes = B * (9 * n * n * N + 5 * n * N) * 4
+            gbs = bbytes / (bload_us * 1e-6) / 1e9
+            lsp = {"us": round(bload_us, 2), "algorithmic_bytes": bbytes, "bound": "hbm", "achieved": round(gbs, 1),
+                   "unit": "GB/s", "peak": HBM_PEAK_GBS, "frac": round(gbs / HBM_PEAK_GBS, 4),
+                   "frac_of_measured": round(gbs / rates["hbm_read_gbs"], 4), "plan": plan0_info}
         broof = batched_roofline(np, gbdpcg, plan, bres, n, N, float(np.median(bkern_each)), rates)
         e_lam = e_gg = 0.0
         for d, (Sb64, _, gb64, gl64) in zip(systems, host64):
@@ -331,13 +343,7 @@ def main():
                    "all_converged": all(r.status == 0 and r.adj_status == 0 for r in bres),
                    "max_rel_err_vs_fp64_cholesky": {"lambda": e_lam, "g_gamma": e_gg, "systems_checked": B},
                    "kernel_us": spread(np, bkern_each), "roofline": broof,
-                   "load_store_phase": {"us": round(bload_us, 2), "algorithmic_bytes": bbytes, "bound": "hbm",
-                                        "achieved": round(bbytes / (bload_us * 1e-6) / 1e9, 1), "unit": "GB/s",
-                                        "peak": HBM_PEAK_GBS,
-                                        "frac": round(bbytes / (bload_us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
-                                        "frac_of_measured": round(bbytes / (bload_us * 1e-6) / 1e9
-                                                                  / rates["hbm_read_gbs"], 4),
-                                        "plan": plan0_info},
+                   "load_store_phase": lsp,
                    "plan": plan.info(), "tune": plan.tune_report,
                    "cpu_baseline": cpu_batched(np, host64, n, N, tol, max_iter, args.cpu_seconds)}
 
