@@ -190,3 +190,33 @@ def test_graph_capture_replay(gpu):
         assert [r.iters for r in res] == eager_iters
         for d, e in zip(systems, eager):
             assert np.array_equal(d["lam"].cpu().numpy(), e)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_padding_blocks_never_used(gpu, dtype, adjoint):
+    """Batched plans (runtime-n and paired instances) with NaN in every system's
+    padding blocks S_{0,-1}, S_{N-1,N} (and Pinv's) give bit-identical results
+    to zero padding."""
+    import gbdpcg
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    specs = [(14, 40, tol), (6, 17, tol), (12, 33, tol), (14, 8, tol)]
+    out = []
+    for fill in (0.0, np.nan):
+        systems, _ = make_batch(specs, dtype, adjoint=adjoint, seed0=9400)
+        for d in systems:
+            n, N = d["n"], d["N"]
+            bs = 3 * n * n
+            for key in ("S", "Pinv"):
+                d[key][:n * n] = fill
+                d[key][(N - 1) * bs + 2 * n * n:N * bs] = fill
+        plan = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+        plan.run()
+        res = plan.results()
+        assert all(r.status == gbdpcg.CONVERGED for r in res)
+        out.append(([r.iters for r in res], [d["lam"].cpu().numpy() for d in systems],
+                    [d["ggamma"].cpu().numpy() for d in systems] if adjoint else []))
+        plan.close()
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1] + out[0][2], out[1][1] + out[1][2]):
+        assert np.array_equal(a, b)

@@ -234,3 +234,34 @@ def test_async_solve_graph_capture(gpu):
             assert np.array_equal(lam.cpu().numpy(), e)
             r = gbdpcg.Result.from_buffer_copy(rec.cpu().numpy().tobytes())
             assert r.status == gbdpcg.CONVERGED
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kernel", [1, 2, 3])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_padding_blocks_never_used(gpu, dtype, kernel, recurrence):
+    """The padding blocks S_{0,-1} and S_{N-1,N} (and Pinv's) hold NaN: the solve
+    is bit-identical to the one with zero padding (include/pcg.cuh:105-106 leaves
+    them unread; the kernels may load them but must discard them)."""
+    import gbdpcg
+    n, N = 14, 40
+    if kernel == 3 and recurrence == 0:
+        pytest.skip("the row-broadcast layout runs the fused recurrence only")
+    S, P, g = lqr_system(n, N, 123, dtype)
+    bs = 3 * n * n
+    Sn, Pn = S.copy(), P.copy()
+    for A in (Sn, Pn):
+        A[:n * n] = np.nan
+        A[(N - 1) * bs + 2 * n * n:N * bs] = np.nan
+    kw = dict(tol=1e-5 if dtype == np.float32 else 1e-10, max_iter=173, exit_mode=1, recurrence=recurrence,
+              kernel=kernel)
+    lam0 = np.zeros(n * N, dtype)
+    try:
+        lam_z, res_z = gpu_solve(S, P, g, lam0, n, N, dtype, **kw)
+    except gbdpcg.GbdpcgError as e:
+        assert e.code in (gbdpcg.ERR_UNSUPPORTED, gbdpcg.ERR_TOO_LARGE)
+        pytest.skip(f"layout not available: {e}")
+    lam_n, res_n = gpu_solve(Sn, Pn, g, lam0, n, N, dtype, **kw)
+    assert res_z.status == gbdpcg.CONVERGED and res_n.status == gbdpcg.CONVERGED
+    assert res_n.iters == res_z.iters
+    assert np.array_equal(lam_n, lam_z)
