@@ -265,3 +265,44 @@ def test_padding_blocks_never_used(gpu, dtype, kernel, recurrence):
     assert res_z.status == gbdpcg.CONVERGED and res_n.status == gbdpcg.CONVERGED
     assert res_n.iters == res_z.iters
     assert np.array_equal(lam_n, lam_z)
+
+
+def test_concurrent_streams(gpu):
+    """Two host threads, each solving on its own stream (and so in its own
+    workspace and exchange records), concurrently: every result is bit-identical
+    to the same solve run alone."""
+    import threading
+    torch = _torch()
+    import gbdpcg
+    n, N, dtype = 14, 64, np.float32
+    kw = dict(tol=1e-5, max_iter=173, exit_mode=1, recurrence=1)
+    systems = [lqr_system(n, N, 300 + i, dtype) for i in range(2)]
+    ref = [gpu_solve(S, P, g, np.zeros(n * N, dtype), n, N, dtype, **kw) for S, P, g in systems]
+    out, errors = [None, None], []
+
+    def worker(i):
+        try:
+            S, P, g = systems[i]
+            st = torch.cuda.Stream()
+            with torch.cuda.stream(st):
+                dS, dP, dg = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (S, P, g))
+                res = []
+                for _ in range(30):
+                    lam = torch.zeros(n * N, device="cuda")
+                    r = gbdpcg.solve(dS, dg, lam, n, N, dP, stream=st, **kw)
+                    res.append((r.iters, lam.cpu().numpy()))
+            out[i] = res
+        except Exception as e:  # reported below, in the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=100)
+    assert not errors, errors
+    for i in range(2):
+        lam_ref, res_ref = ref[i]
+        assert out[i] is not None and len(out[i]) == 30
+        for iters, lam in out[i]:
+            assert iters == res_ref.iters and np.array_equal(lam, lam_ref)
