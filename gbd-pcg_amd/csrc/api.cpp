@@ -53,12 +53,12 @@ constexpr int kXcds = 8;                    // gfx950: 8 XCDs, workgroups dealt 
 
 static size_t tsize(int dtype) { return dtype == GBDPCG_F64 ? 8 : 4; }
 
-uint32_t exchange_flags() {
+uint32_t exchange_flags(const gbdpcg_config *cfg) {
   static const uint32_t f = [] {
     const char *e = getenv("GBDPCG_DEVICE_SCOPE_EXCHANGE");
     return (e && *e && strcmp(e, "0") != 0) ? (uint32_t)KF_NOXCD : 0u;
   }();
-  return f;
+  return f | ((cfg && cfg->exchange_scope == GBDPCG_EXCHANGE_DEVICE) ? (uint32_t)KF_NOXCD : 0u);
 }
 static bool valid_dtype(int dtype) { return dtype == GBDPCG_F32 || dtype == GBDPCG_F64; }
 
@@ -126,19 +126,22 @@ struct SlotBuf {
 };
 
 struct Workspace {
-  std::mutex mu;
+  std::mutex mu;  // held by a call from its first enqueue to its last read of the fields below
   int device = 0;
   hipStream_t stream = nullptr;
   std::map<uint64_t, SlotBuf> slots;  // keyed by layout signature
-  gbdpcg_result *d_result = nullptr;
-  int *d_err = nullptr;
-  int *d_perr = nullptr;
+  gbdpcg_result *d_result = nullptr;  // device: record of async solves given no record of their own
+  // pinned, device-visible host memory written by the kernels in place (no copy
+  // back): the synchronous solves' record and the error words, [0] = an exchange
+  // wait timed out, [1] = a singular diagonal block in the preconditioner
   gbdpcg_result *h_result = nullptr;
   int *h_err = nullptr;
   void *d_pinv = nullptr;
   size_t pinv_bytes = 0;
-  void *d_io = nullptr;
-  size_t io_bytes = 0;
+  // pinned, device-visible staging of the host-pointer solves: S | Pinv | gamma |
+  // lambda, read and written by the kernels in place
+  void *h_stage = nullptr;
+  size_t stage_bytes = 0;
 };
 
 static std::mutex g_ws_mu;
@@ -166,11 +169,14 @@ static int get_workspace(int dev, hipStream_t stream, Workspace **out) {
   ws->device = dev;
   ws->stream = stream;
   HIPCHK(hipMalloc(&ws->d_result, sizeof(gbdpcg_result)));
-  HIPCHK(hipMalloc(&ws->d_err, 2 * sizeof(int)));
-  ws->d_perr = ws->d_err + 1;
-  HIPCHK(hipMemset(ws->d_err, 0, 2 * sizeof(int)));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ws->h_result), sizeof(gbdpcg_result), hipHostMallocDefault));
-  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ws->h_err), 2 * sizeof(int), hipHostMallocDefault));
+  // coherent host memory: never cached on the device, so what the kernels write is
+  // in host memory once the stream is synchronised, and host writes are seen by
+  // the next launch
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ws->h_result), sizeof(gbdpcg_result),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  HIPCHK(hipHostMalloc(reinterpret_cast<void **>(&ws->h_err), 2 * sizeof(int),
+                       hipHostMallocMapped | hipHostMallocCoherent));
+  ws->h_err[0] = ws->h_err[1] = 0;
   *out = ws.get();
   g_ws[key] = std::move(ws);
   return GBDPCG_OK;
@@ -206,6 +212,19 @@ static int grow(void **p, size_t *cap, size_t need, hipStream_t s) {
   *cap = 0;
   HIPCHK(hipMalloc(p, need));
   *cap = need;
+  return GBDPCG_OK;
+}
+
+// Pinned host staging of at least `need` bytes (the caller holds the workspace
+// lock, and every earlier use of the buffer has completed: the host-pointer
+// solves synchronise before they return).
+static int grow_pinned(Workspace *ws, size_t need) {
+  if (ws->stage_bytes >= need) return GBDPCG_OK;
+  if (ws->h_stage) HIPCHK(hipHostFree(ws->h_stage));
+  ws->h_stage = nullptr;
+  ws->stage_bytes = 0;
+  HIPCHK(hipHostMalloc(&ws->h_stage, need, hipHostMallocMapped | hipHostMallocCoherent));
+  ws->stage_bytes = need;
   return GBDPCG_OK;
 }
 
@@ -438,30 +457,33 @@ static int validate_cfg(const gbdpcg_config *cfg) {
     set_error("bad precond %d", cfg->precond);
     return GBDPCG_ERR_INVALID;
   }
+  if (cfg->exchange_scope != GBDPCG_EXCHANGE_AUTO && cfg->exchange_scope != GBDPCG_EXCHANGE_DEVICE) {
+    set_error("bad exchange_scope %u", cfg->exchange_scope);
+    return GBDPCG_ERR_INVALID;
+  }
   return GBDPCG_OK;
 }
 
+// The error words live in pinned host memory; read after the stream is
+// synchronised, cleared by host stores (no kernel of this workspace is running).
 static int check_error_words(Workspace *ws, uint64_t sig) {
   if (ws->h_err[0]) {
     ws->slots[sig].dirty = true;
-    HIPCHK(hipMemsetAsync(ws->d_err, 0, sizeof(int), ws->stream));
-    HIPCHK(hipStreamSynchronize(ws->stream));
+    ws->h_err[0] = 0;
     set_error("a cross-workgroup exchange wait timed out (workgroups not co-resident?)");
     return GBDPCG_ERR_TIMEOUT;
   }
   if (ws->h_err[1]) {
-    HIPCHK(hipMemsetAsync(ws->d_perr, 0, sizeof(int), ws->stream));
-    HIPCHK(hipStreamSynchronize(ws->stream));
+    ws->h_err[1] = 0;
     set_error("preconditioner: a diagonal block of S is singular");
     return GBDPCG_ERR_INVALID;
   }
   return GBDPCG_OK;
 }
 
-// Core of every single-system device entry point.
-static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, const void *Pinv, const void *gamma,
-                             void *lambda, const void *glambda, void *ggamma, void *gS, uint32_t flags,
-                             const gbdpcg_config *cfg, bool sync, gbdpcg_result *h_out, gbdpcg_result *d_out) {
+static int check_args(int dtype, uint32_t n, uint32_t N, const void *S, const void *gamma, const void *lambda,
+                      const void *glambda, const void *ggamma, const void *gS, uint32_t flags,
+                      const gbdpcg_config *cfg) {
   int rc = validate_cfg(cfg);
   if (rc < 0) return rc;
   if (!valid_dtype(dtype) || n == 0 || N == 0 || !S || !lambda) {
@@ -476,27 +498,32 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
     set_error("adjoint needs glambda, ggamma and gS");
     return GBDPCG_ERR_INVALID;
   }
-  int dev = 0;
-  HIPCHK(hipGetDevice(&dev));
-  hipStream_t stream = static_cast<hipStream_t>(cfg->stream);
-  Workspace *ws = nullptr;
-  rc = get_workspace(dev, stream, &ws);
-  if (rc < 0) return rc;
-  std::lock_guard<std::mutex> lk(ws->mu);
+  return GBDPCG_OK;
+}
 
+// Core of every single-system entry point; the caller holds ws->mu from here to
+// its last use of the workspace's buffers.  sync: wait for the stream, check the
+// error words and return the iteration count (the kernel writes the record
+// straight to pinned host memory); else the record goes to d_out (or the
+// workspace's device record) and 0 is returned after the launch.
+static int solve_locked(Workspace *ws, int dtype, uint32_t n, uint32_t N, const void *S, const void *Pinv,
+                        const void *gamma, void *lambda, const void *glambda, void *ggamma, void *gS,
+                        uint32_t flags, const gbdpcg_config *cfg, bool sync, gbdpcg_result *h_out,
+                        gbdpcg_result *d_out) {
+  hipStream_t stream = ws->stream;
   bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED;
   if (fused && N < 2) fused = false;  // one knot: the reference recurrence
   // forward + adjoint on a register kernel (n <= 14): both solves in one loop
   const bool pair = fused && (flags & KF_FWD) && (flags & KF_ADJ) && n <= 14;
   Layout L;
-  rc = choose_layout(dev, dtype, n, N, fused, cfg, &L, pair);
+  int rc = choose_layout(ws->device, dtype, n, N, fused, cfg, &L, pair);
   if (rc < 0) return rc;
 
   if (Pinv == nullptr) {
     const size_t bytes = (size_t)3 * n * n * N * tsize(dtype);
     rc = grow(&ws->d_pinv, &ws->pinv_bytes, bytes, stream);
     if (rc < 0) return rc;
-    HIPCHK(precond_launch(dtype, n, N, S, ws->d_pinv, cfg->precond, ws->d_perr, stream));
+    HIPCHK(precond_launch(dtype, n, N, S, ws->d_pinv, cfg->precond, ws->h_err + 1, stream));
     Pinv = ws->d_pinv;
   }
   const size_t words = 2 * (size_t)L.G * rec_words(n, (uint32_t)(tsize(dtype) / 4));
@@ -514,15 +541,15 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
   one.ggamma = ggamma;
   one.gS = gS;
   one.slots = slots;
-  one.result = d_out ? d_out : ws->d_result;
-  one.err = ws->d_err;
+  one.result = sync ? ws->h_result : (d_out ? d_out : ws->d_result);
+  one.err = ws->h_err;
   one.tol = cfg->exit_tol;
   one.n = n;
   one.N = N;
   one.G = L.G;
   one.max_iter = cfg->max_iter;
   one.flags = flags | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
-              exchange_flags();
+              exchange_flags(cfg);
   if (L.reg) {
     HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream,
                       RegVariant{fused, L.bc, L.pair}));
@@ -531,20 +558,41 @@ static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, c
   }
   if (!sync) return GBDPCG_OK;
 
-  HIPCHK(hipMemcpyAsync(ws->h_result, one.result, sizeof(gbdpcg_result), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipMemcpyAsync(ws->h_err, ws->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
   HIPCHK(hipStreamSynchronize(stream));
   rc = check_error_words(ws, sig);
   if (rc < 0) return rc;
-  if (h_out) *h_out = *ws->h_result;
-  if (ws->h_result->status == GBDPCG_TIMED_OUT || ws->h_result->adj_status == GBDPCG_TIMED_OUT) {
+  const gbdpcg_result res = *ws->h_result;
+  if (h_out) *h_out = res;
+  if (res.status == GBDPCG_TIMED_OUT || res.adj_status == GBDPCG_TIMED_OUT) {
     ws->slots[sig].dirty = true;
     set_error("a cross-workgroup exchange wait timed out");
     return GBDPCG_ERR_TIMEOUT;
   }
-  return (flags & KF_FWD) ? ws->h_result->iters : ws->h_result->adj_iters;
+  return (flags & KF_FWD) ? res.iters : res.adj_iters;
 }
 
+// Every single-system device entry point: the workspace of (current device,
+// stream) is locked for the whole call, so calls from several host threads on
+// one stream run one after the other.
+static int solve_device_impl(int dtype, uint32_t n, uint32_t N, const void *S, const void *Pinv, const void *gamma,
+                             void *lambda, const void *glambda, void *ggamma, void *gS, uint32_t flags,
+                             const gbdpcg_config *cfg, bool sync, gbdpcg_result *h_out, gbdpcg_result *d_out) {
+  int rc = check_args(dtype, n, N, S, gamma, lambda, glambda, ggamma, gS, flags, cfg);
+  if (rc < 0) return rc;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  Workspace *ws = nullptr;
+  rc = get_workspace(dev, static_cast<hipStream_t>(cfg->stream), &ws);
+  if (rc < 0) return rc;
+  std::lock_guard<std::mutex> lk(ws->mu);
+  return solve_locked(ws, dtype, n, N, S, Pinv, gamma, lambda, glambda, ggamma, gS, flags, cfg, sync, h_out, d_out);
+}
+
+// The host-pointer drop-in: the inputs are copied into the workspace's pinned
+// staging, which the kernels read in place (no device copy of the inputs), and
+// the kernel writes lambda there; one stream synchronisation, then lambda is
+// copied out.  The workspace stays locked from the first copy in to the last
+// copy out.
 template <typename T>
 static int solve_host_impl(int dtype, const T *h_S, const T *h_Pinv, const T *h_gamma, T *h_lambda, uint32_t n,
                            uint32_t N, const gbdpcg_config *cfg) {
@@ -557,29 +605,29 @@ static int solve_host_impl(int dtype, const T *h_S, const T *h_Pinv, const T *h_
   int dev = 0;
   HIPCHK(hipGetDevice(&dev));
   hipStream_t stream = static_cast<hipStream_t>(cfg->stream);
+  if (capturing(stream)) {
+    set_error("the host-pointer solve synchronises: it cannot run inside graph capture");
+    return GBDPCG_ERR_INVALID;
+  }
   Workspace *ws = nullptr;
   rc = get_workspace(dev, stream, &ws);
   if (rc < 0) return rc;
   const size_t mat = (size_t)3 * n * n * N, vec = (size_t)n * N;
-  T *d_S, *d_P, *d_g, *d_l;
-  {
-    std::lock_guard<std::mutex> lk(ws->mu);
-    rc = grow(&ws->d_io, &ws->io_bytes, (2 * mat + 2 * vec) * sizeof(T), stream);
-    if (rc < 0) return rc;
-    d_S = static_cast<T *>(ws->d_io);
-    d_P = d_S + mat;
-    d_g = d_P + mat;
-    d_l = d_g + vec;
-    HIPCHK(hipMemcpyAsync(d_S, h_S, mat * sizeof(T), hipMemcpyHostToDevice, stream));
-    if (h_Pinv) HIPCHK(hipMemcpyAsync(d_P, h_Pinv, mat * sizeof(T), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_g, h_gamma, vec * sizeof(T), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(d_l, h_lambda, vec * sizeof(T), hipMemcpyHostToDevice, stream));
-  }
-  const int iters = solve_device_impl(dtype, n, N, d_S, h_Pinv ? d_P : nullptr, d_g, d_l, nullptr, nullptr, nullptr,
-                                      KF_FWD, cfg, true, nullptr, nullptr);
+  std::lock_guard<std::mutex> lk(ws->mu);
+  rc = grow_pinned(ws, (2 * mat + 2 * vec) * sizeof(T));
+  if (rc < 0) return rc;
+  T *s_S = static_cast<T *>(ws->h_stage);
+  T *s_P = s_S + mat;
+  T *s_g = s_P + mat;
+  T *s_l = s_g + vec;
+  memcpy(s_S, h_S, mat * sizeof(T));
+  if (h_Pinv) memcpy(s_P, h_Pinv, mat * sizeof(T));
+  memcpy(s_g, h_gamma, vec * sizeof(T));
+  memcpy(s_l, h_lambda, vec * sizeof(T));
+  const int iters = solve_locked(ws, dtype, n, N, s_S, h_Pinv ? s_P : nullptr, s_g, s_l, nullptr, nullptr, nullptr,
+                                 KF_FWD, cfg, true, nullptr, nullptr);
   if (iters < 0) return iters;
-  HIPCHK(hipMemcpyAsync(h_lambda, d_l, vec * sizeof(T), hipMemcpyDeviceToHost, stream));
-  HIPCHK(hipStreamSynchronize(stream));
+  memcpy(h_lambda, s_l, vec * sizeof(T));
   return iters;
 }
 
@@ -658,7 +706,7 @@ int gbdpcg_exchange_floor(uint32_t wgs, uint32_t threads, uint32_t rounds, doubl
   }
   // the solvers' placement: all ranks on one XCD when they fit there
   const uint32_t stride = wgs <= (uint32_t)(di.cus / kXcds) * (uint32_t)occ ? (uint32_t)kXcds : 1u;
-  const uint32_t allow_xcd = (exchange_flags() & KF_NOXCD) ? 0u : 1u;
+  const uint32_t allow_xcd = (exchange_flags(nullptr) & KF_NOXCD) ? 0u : 1u;
   uint64_t *slots = nullptr;
   int *out = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -978,7 +1026,7 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     one.G = L.G;
     one.max_iter = cfg->max_iter;
     one.flags = KF_FWD | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
-                exchange_flags();
+                exchange_flags(cfg);
     double total = 0.0;
     for (uint32_t t = 0; t <= trials; ++t) {
       if ((e = hipMemsetAsync(lam, 0, (size_t)n * N * ts, st)) != hipSuccess) return fail("hipMemsetAsync", e);
@@ -1057,12 +1105,10 @@ int gbdpcg_precond_device(int dtype, uint32_t n, uint32_t N, const void *d_S, vo
   rc = get_workspace(dev, s, &ws);
   if (rc < 0) return rc;
   std::lock_guard<std::mutex> lk(ws->mu);
-  HIPCHK(precond_launch(dtype, n, N, d_S, d_Pinv, kind, ws->d_perr, s));
-  HIPCHK(hipMemcpyAsync(ws->h_err, ws->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(precond_launch(dtype, n, N, d_S, d_Pinv, kind, ws->h_err + 1, s));
   HIPCHK(hipStreamSynchronize(s));
   if (ws->h_err[1]) {
-    HIPCHK(hipMemsetAsync(ws->d_perr, 0, sizeof(int), s));
-    HIPCHK(hipStreamSynchronize(s));
+    ws->h_err[1] = 0;
     set_error("gbdpcg_precond_device: a diagonal block of S is singular");
     return GBDPCG_ERR_INVALID;
   }

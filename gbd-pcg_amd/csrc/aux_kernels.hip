@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(kAuxThreads) precond_kernel(const T *__restric
   if (ok && ss && k > 0) ok = invert_diag(S, k - 1, n, aug, fac, piv, inv_prev);
   if (ok && ss && k < N - 1) ok = invert_diag(S, k + 1, n, aug, fac, piv, inv_next);
   if (!ok) {
-    if (threadIdx.x == 0) atomicOr(err, 1);
+    if (threadIdx.x == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     for (int i = threadIdx.x; i < (int)bs; i += blockDim.x) row[i] = (T)0;
     return;
   }
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
   }
   T *row = P + (size_t)k * bs;
   if (!ok) {
-    if (lane == 0) atomicOr(err, 1);
+    if (lane == 0) __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     for (int i = lane; i < (int)bs; i += 64) row[i] = (T)0;
     return;
   }

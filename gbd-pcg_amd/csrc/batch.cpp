@@ -278,7 +278,8 @@ extern "C" {
 
 int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc *systems, uint32_t count,
                         const gbdpcg_config *cfg, uint32_t lds_budget_bytes) {
-  if (!out || !systems || count == 0 || !cfg || (dtype != GBDPCG_F32 && dtype != GBDPCG_F64)) {
+  if (!out || !systems || count == 0 || !cfg || (dtype != GBDPCG_F32 && dtype != GBDPCG_F64) ||
+      (cfg->exchange_scope != GBDPCG_EXCHANGE_AUTO && cfg->exchange_scope != GBDPCG_EXCHANGE_DEVICE)) {
     set_error("gbdpcg_batch_create: invalid argument");
     return GBDPCG_ERR_INVALID;
   }
@@ -459,7 +460,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     k.max_iter = d.max_iter;
     k.flags = ((d.flags & GBDPCG_SYS_FORWARD) ? KF_FWD : 0u) | ((d.flags & GBDPCG_SYS_ADJOINT) ? KF_ADJ : 0u) |
               (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused[i] ? KF_FUSED : 0u) |
-              exchange_flags();
+              exchange_flags(cfg);
   }
   if ((e = hipMemset(plan->d_slots, 0, words * sizeof(uint64_t))) != hipSuccess ||
       (e = hipMemset(plan->d_results, 0, count * sizeof(gbdpcg_result))) != hipSuccess ||

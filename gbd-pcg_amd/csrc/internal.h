@@ -39,7 +39,8 @@ struct KSys {
   void *gS;
   uint64_t *slots;        // exchange words of this system: 2 * G * rec_words
   gbdpcg_result *result;  // result record (may be null)
-  int *err;               // launch-wide error word: bit 0 set on a timed-out wait
+  int *err;               // launch-wide error word: set to 1 on a timed-out wait (a plain store: it may
+                          // be pinned host memory)
   double tol;
   uint32_t n, N, G, max_iter;
   uint32_t flags;
@@ -184,9 +185,11 @@ void tune_forget_all();
 }  // namespace gbdpcg
 struct gbdpcg_batch;
 namespace gbdpcg {
-// api.cpp: KF_NOXCD when the environment variable GBDPCG_DEVICE_SCOPE_EXCHANGE is
-// set to a non-empty value other than "0" (read once), else 0
-uint32_t exchange_flags();
+// api.cpp: KF_NOXCD when cfg asks for device-scope exchange stores
+// (cfg->exchange_scope == GBDPCG_EXCHANGE_DEVICE) or the environment variable
+// GBDPCG_DEVICE_SCOPE_EXCHANGE is set to a non-empty value other than "0" (read
+// once); else 0.  cfg may be null (the environment alone decides).
+uint32_t exchange_flags(const gbdpcg_config *cfg);
 // batch.cpp: properties of a built plan used by the tuner
 bool batch_uses_registers(const gbdpcg_batch *plan);
 int batch_kernel(const gbdpcg_batch *plan);  // GBDPCG_KERNEL_LDS / _REGISTER / _ROWCAST

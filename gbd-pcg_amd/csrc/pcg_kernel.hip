@@ -657,7 +657,7 @@ __device__ __forceinline__ void run_item(const KSys &d, uint32_t rank, T *lds) {
   }
   // any rank that timed out flags the launch, so the host never misses a failure
   if (timed_out && threadIdx.x == 0 && d.err != nullptr) {
-    __hip_atomic_fetch_or(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 

@@ -1117,7 +1117,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
     *d.result = res;
   }
   if (timed_out && t == 0 && d.err != nullptr)
-    __hip_atomic_fetch_or(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Forward and adjoint solves of one system iterated together (fused recurrence;
@@ -1457,7 +1457,7 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
     *d.result = res;
   }
   if (timed_out && t == 0 && d.err != nullptr)
-    __hip_atomic_fetch_or(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 }  // namespace

@@ -30,6 +30,7 @@ EXIT_ABS_ETA, EXIT_REL_RESID = 0, 1
 PRECOND_JACOBI, PRECOND_SS = 1, 2
 RECUR_REFERENCE, RECUR_FUSED = 0, 1
 KERNEL_AUTO, KERNEL_LDS, KERNEL_REGISTER, KERNEL_ROWCAST = 0, 1, 2, 3
+EXCHANGE_AUTO, EXCHANGE_DEVICE = 0, 1
 SYS_FORWARD, SYS_ADJOINT = 1, 2
 
 _ERRNAMES = {ERR_INVALID: "INVALID", ERR_TOO_LARGE: "TOO_LARGE", ERR_DEVICE: "DEVICE", ERR_TIMEOUT: "TIMEOUT",
@@ -47,7 +48,8 @@ class Config(ctypes.Structure):
                 ("block", ctypes.c_uint32 * 3), ("empty_pinv", ctypes.c_int32), ("exit_mode", ctypes.c_int32),
                 ("precond", ctypes.c_int32), ("recurrence", ctypes.c_int32), ("stream", ctypes.c_void_p),
                 ("kernel", ctypes.c_uint32), ("knots_per_wg", ctypes.c_uint32), ("threads", ctypes.c_uint32),
-                ("batch_split", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("batch_split", ctypes.c_uint32), ("exchange_scope", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
 
 
 class Result(ctypes.Structure):
@@ -175,7 +177,7 @@ def _dtype_code(dt) -> int:
 def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS_ETA,
                 recurrence: int = RECUR_REFERENCE, precond: int = PRECOND_SS, wgs: int = 0,
                 stream: Optional[int] = None, kernel: int = KERNEL_AUTO, knots_per_wg: int = 0,
-                threads: int = 0, batch_split: bool = False) -> Config:
+                threads: int = 0, batch_split: bool = False, exchange_scope: int = EXCHANGE_AUTO) -> Config:
     cfg = Config()
     lib().gbdpcg_config_init(ctypes.byref(cfg))
     cfg.exit_tol = float(tol)
@@ -189,6 +191,7 @@ def make_config(tol: float = 1e-6, max_iter: int = 25, exit_mode: int = EXIT_ABS
     cfg.knots_per_wg = int(knots_per_wg)
     cfg.threads = int(threads)
     cfg.batch_split = 1 if batch_split else 0
+    cfg.exchange_scope = int(exchange_scope)
     return cfg
 
 
@@ -209,18 +212,52 @@ def _dev(t, name):
     return t.data_ptr()
 
 
+def _check_system(S, n: int, N: int, mats=(), vecs=(), where: str = "") -> int:
+    """Checks one system's device tensors before anything is launched: every
+    tensor (given as (name, tensor) pairs; None entries are skipped) is a
+    contiguous device tensor on S's device with S's dtype, S and the other cbtd
+    arrays hold 3 n^2 N elements and the vectors n N.  Returns the dtype code.
+    A short or mistyped buffer would otherwise be read or written out of bounds
+    by the kernels."""
+    if int(n) <= 0 or int(N) <= 0:
+        raise ValueError(f"{where}n and N must be positive (got n={n}, N={N})")
+    dt = _dtype_code(S.dtype)
+    want = {"mat": 3 * int(n) * int(n) * int(N), "vec": int(n) * int(N)}
+    for kind, group in (("mat", (("S", S),) + tuple(mats)), ("vec", tuple(vecs))):
+        for name, t in group:
+            if t is None:
+                continue
+            if t.dtype != S.dtype:
+                raise ValueError(f"{where}{name} has dtype {t.dtype}, S has {S.dtype}")
+            if t.numel() != want[kind]:
+                raise ValueError(f"{where}{name} has {t.numel()} elements, expected {want[kind]} "
+                                 f"({'3 n^2 N' if kind == 'mat' else 'n N'} with n={n}, N={N})")
+            if t.device != S.device:
+                raise ValueError(f"{where}{name} is on {t.device}, S is on {S.device}")
+            _dev(t, where + name)
+    return dt
+
+
+def _check_result_buffer(d_result):
+    if d_result is None:
+        return
+    _dev(d_result, "d_result")
+    if d_result.numel() * d_result.element_size() < RESULT_BYTES:
+        raise ValueError(f"d_result must hold at least {RESULT_BYTES} bytes")
+
+
 # ---------------------------------------------------------------------------
 # single-system solves (torch device tensors)
 # ---------------------------------------------------------------------------
 
 def solve(S, gamma, lam, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, exit_mode=EXIT_ABS_ETA,
           recurrence=RECUR_REFERENCE, precond=PRECOND_SS, wgs=0, stream=None, kernel=KERNEL_AUTO,
-          threads=0) -> Result:
+          threads=0, exchange_scope=EXCHANGE_AUTO) -> Result:
     """Solves S lam = gamma in place (lam holds the initial guess).  Waits for the
     stream; returns the result record."""
-    dt = _dtype_code(S.dtype)
+    dt = _check_system(S, n, N, mats=(("Pinv", Pinv),), vecs=(("gamma", gamma), ("lam", lam)))
     cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel,
-                      threads=threads)
+                      threads=threads, exchange_scope=exchange_scope)
     res = Result()
     f = lib().gbdpcg_solve_device_f64 if dt == F64 else lib().gbdpcg_solve_device_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
@@ -230,12 +267,13 @@ def solve(S, gamma, lam, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, ex
 
 def solve_async(S, gamma, lam, n: int, N: int, Pinv=None, d_result=None, *, tol=1e-6, max_iter=25,
                 exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, precond=PRECOND_SS, wgs=0, stream=None,
-                kernel=KERNEL_AUTO, threads=0) -> None:
+                kernel=KERNEL_AUTO, threads=0, exchange_scope=EXCHANGE_AUTO) -> None:
     """Launches a solve without waiting; the result record goes to d_result (a
     device uint8 tensor of RESULT_BYTES bytes) when given."""
-    dt = _dtype_code(S.dtype)
+    dt = _check_system(S, n, N, mats=(("Pinv", Pinv),), vecs=(("gamma", gamma), ("lam", lam)))
+    _check_result_buffer(d_result)
     cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel,
-                      threads=threads)
+                      threads=threads, exchange_scope=exchange_scope)
     f = lib().gbdpcg_solve_device_async_f64 if dt == F64 else lib().gbdpcg_solve_device_async_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
             _dev(d_result, "d_result")))
@@ -243,11 +281,15 @@ def solve_async(S, gamma, lam, n: int, N: int, Pinv=None, d_result=None, *, tol=
 
 def adjoint(S, gamma, lam, glam, ggamma, gS, n: int, N: int, Pinv, *, forward=True, tol=1e-6, max_iter=25,
             exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, wgs=0, stream=None, kernel=KERNEL_AUTO,
-            threads=0) -> Result:
+            threads=0, exchange_scope=EXCHANGE_AUTO) -> Result:
     """(Forward +) adjoint solve; writes ggamma and gS (cbtd).  Waits for the stream."""
-    dt = _dtype_code(S.dtype)
+    if Pinv is None or glam is None or ggamma is None or gS is None or (forward and gamma is None):
+        raise ValueError("adjoint needs Pinv, glam, ggamma, gS (and gamma when forward)")
+    dt = _check_system(S, n, N, mats=(("Pinv", Pinv), ("gS", gS)),
+                       vecs=(("gamma", gamma if forward else None), ("lam", lam), ("glam", glam),
+                             ("ggamma", ggamma)))
     cfg = make_config(tol, max_iter, exit_mode, recurrence, PRECOND_SS, wgs, _stream_handle(stream), kernel,
-                      threads=threads)
+                      threads=threads, exchange_scope=exchange_scope)
     res = Result()
     f = lib().gbdpcg_adjoint_device_f64 if dt == F64 else lib().gbdpcg_adjoint_device_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma") if forward else None, _dev(lam, "lam"),
@@ -257,6 +299,7 @@ def adjoint(S, gamma, lam, glam, ggamma, gS, n: int, N: int, Pinv, *, forward=Tr
 
 
 def precond_device(S, Pinv, n: int, N: int, kind: int = PRECOND_SS, stream=None) -> None:
+    _check_system(S, n, N, mats=(("Pinv", Pinv),))
     check(lib().gbdpcg_precond_device(_dtype_code(S.dtype), n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), kind,
                                       _stream_handle(stream)))
 
@@ -268,6 +311,10 @@ def lqr_schur_device(A, B, Qd, Rd, S, n: int, m: int, N: int, stream=None) -> No
 
 
 def warm_start_device(lam, gamma, n: int, N: int, noise=None, stream=None) -> None:
+    for name, t in (("lam", lam), ("gamma", gamma), ("noise", noise)):
+        if t is not None and (t.dtype != lam.dtype or t.numel() != n * N):
+            raise ValueError(f"{name} must have lam's dtype and n N = {n * N} elements")
+        _dev(t, name)
     check(lib().gbdpcg_warm_start_device(_dtype_code(lam.dtype), n, N, _dev(lam, "lam"), _dev(gamma, "gamma"),
                                          _dev(noise, "noise"), _stream_handle(stream)))
 
@@ -285,9 +332,18 @@ def solve_host(S: np.ndarray, gamma: np.ndarray, lam: np.ndarray, n: int, N: int
                precond=PRECOND_SS, wgs=0) -> int:
     """The reference's host-pointer call: lam (numpy, in/out).  Returns iterations."""
     dt = _dtype_code(S.dtype)
-    for a in (S, gamma, lam) + ((Pinv,) if Pinv is not None else ()):
+    if int(n) <= 0 or int(N) <= 0:
+        raise ValueError(f"n and N must be positive (got n={n}, N={N})")
+    for name, a, size in (("S", S, 3 * n * n * N), ("gamma", gamma, n * N), ("lam", lam, n * N),
+                          ("Pinv", Pinv, 3 * n * n * N)):
+        if a is None:
+            continue
         if not a.flags["C_CONTIGUOUS"] or a.dtype != S.dtype:
             raise ValueError("arrays must be C-contiguous and share S's dtype")
+        if a.size != size:
+            raise ValueError(f"{name} has {a.size} elements, expected {size} (n={n}, N={N})")
+    if not lam.flags["WRITEABLE"]:
+        raise ValueError("lam is written in place: it must be writeable")
     cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, None)
     f = lib().gbdpcg_solve_f64 if dt == F64 else lib().gbdpcg_solve_f32
     return check(f(_np_ptr(S), _np_ptr(Pinv), _np_ptr(gamma), _np_ptr(lam), n, N, ctypes.byref(cfg)))
@@ -367,6 +423,7 @@ def tune_single(S, gamma, n: int, N: int, Pinv=None, *, tol=1e-6, max_iter=25, e
                 recurrence=RECUR_REFERENCE, precond=PRECOND_SS, kernel=KERNEL_AUTO, trials=5, stream=None) -> dict:
     """Times the candidate layouts of (n, N) on these inputs and makes the fastest
     the default for later automatic solves of this shape.  Returns the report."""
+    _check_system(S, n, N, mats=(("Pinv", Pinv),), vecs=(("gamma", gamma),))
     cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, 0, _stream_handle(stream), kernel)
     rep = TuneReport()
     check(lib().gbdpcg_tune_single(_dtype_code(S.dtype), n, N, _dev(S, "S"), _dev(Pinv, "Pinv"),
@@ -407,13 +464,22 @@ class Batch:
     def __init__(self, systems: Sequence[dict], *, exit_mode=EXIT_REL_RESID, recurrence=RECUR_REFERENCE,
                  lds_budget: int = 0, tol=1e-5, max_iter=173, knots_per_wg: int = 0, threads: int = 0,
                  kernel: int = KERNEL_AUTO, autotune: bool = False, trials: int = 3, stream=None,
-                 split_launches: bool = False):
+                 split_launches: bool = False, exchange_scope: int = EXCHANGE_AUTO):
         if not systems:
             raise ValueError("empty batch")
         self._keep = list(systems)
         dt = _dtype_code(systems[0]["S"].dtype)
         arr = (SystemDesc * len(systems))()
         for i, s in enumerate(systems):
+            fwd, adj = s.get("forward", True), s.get("adjoint", False)
+            if s.get("Pinv") is None or (fwd and s.get("gamma") is None) or \
+                    (adj and (s.get("glam") is None or s.get("ggamma") is None or s.get("gS") is None)):
+                raise ValueError(f"system {i}: needs Pinv, gamma (forward) and glam, ggamma, gS (adjoint)")
+            if s["S"].dtype != systems[0]["S"].dtype or s["S"].device != systems[0]["S"].device:
+                raise ValueError(f"system {i}: every system of a batch shares the first one's dtype and device")
+            _check_system(s["S"], s["n"], s["N"], mats=(("Pinv", s["Pinv"]), ("gS", s.get("gS"))),
+                          vecs=(("gamma", s.get("gamma")), ("lam", s["lam"]), ("glam", s.get("glam")),
+                                ("ggamma", s.get("ggamma"))), where=f"system {i}: ")
             d = arr[i]
             d.S, d.Pinv, d.gamma, d.lam = _dev(s["S"], "S"), _dev(s["Pinv"], "Pinv"), _dev(s.get("gamma"), "gamma"), \
                 _dev(s["lam"], "lam")
@@ -425,7 +491,8 @@ class Batch:
             d.flags = (SYS_FORWARD if s.get("forward", True) else 0) | (SYS_ADJOINT if s.get("adjoint", False) else 0)
             d.wgs = int(s.get("wgs", 0))
         cfg = make_config(exit_mode=exit_mode, recurrence=recurrence, stream=_stream_handle(stream), kernel=kernel,
-                          knots_per_wg=knots_per_wg, threads=threads, batch_split=split_launches)
+                          knots_per_wg=knots_per_wg, threads=threads, batch_split=split_launches,
+                          exchange_scope=exchange_scope)
         h = ctypes.c_void_p()
         self.tune_report = None
         if autotune:

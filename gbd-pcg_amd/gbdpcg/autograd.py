@@ -8,13 +8,24 @@ on chip) and returns
               pattern (cbtd layout, padding blocks zero),
 which is the gradient for perturbations of S that stay symmetric and
 block-tridiagonal.  Pinv gets no gradient (the exact solution does not depend on
-it).  All work runs in the HIP library; tensors must be on the device.
+it).  All work runs in the HIP library; tensors must be on the device.  A
+forward or adjoint solve that does not converge raises RuntimeError.
 """
 from __future__ import annotations
 
 import torch
 
-from . import EXIT_REL_RESID, RECUR_FUSED, adjoint, solve
+from . import CONVERGED, EXIT_REL_RESID, RECUR_FUSED, adjoint, solve
+
+_STATUS = {0: "CONVERGED", 1: "MAX_ITER", 2: "BREAKDOWN", 3: "NONFINITE", 4: "TIMED_OUT"}
+
+
+def _require_converged(what: str, status: int, iters: int) -> None:
+    """An unconverged solve would hand back a wrong lambda or gradient without
+    any sign of it: raise instead."""
+    if status != CONVERGED:
+        raise RuntimeError(f"PCGSolve: the {what} solve ended with status {_STATUS.get(status, status)} after "
+                           f"{iters} iterations (raise max_iter or loosen tol)")
 
 
 class PCGSolve(torch.autograd.Function):
@@ -26,6 +37,7 @@ class PCGSolve(torch.autograd.Function):
         lam = torch.zeros_like(gamma)
         res = solve(S_, gamma.detach().contiguous(), lam, n, N, P_, tol=tol, max_iter=max_iter,
                     exit_mode=exit_mode, recurrence=recurrence)
+        _require_converged("forward", res.status, res.iters)
         ctx.save_for_backward(S_, P_, lam)
         ctx.cfg = (n, N, tol, max_iter, exit_mode, recurrence)
         ctx.forward_result = res
@@ -37,8 +49,9 @@ class PCGSolve(torch.autograd.Function):
         n, N, tol, max_iter, exit_mode, recurrence = ctx.cfg
         gg = torch.zeros_like(lam)
         gS = torch.zeros_like(S_)
-        adjoint(S_, None, lam, glam.contiguous(), gg, gS, n, N, P_, forward=False, tol=tol, max_iter=max_iter,
-                exit_mode=exit_mode, recurrence=recurrence)
+        res = adjoint(S_, None, lam, glam.contiguous(), gg, gS, n, N, P_, forward=False, tol=tol,
+                      max_iter=max_iter, exit_mode=exit_mode, recurrence=recurrence)
+        _require_converged("adjoint", res.adj_status, res.adj_iters)
         return gS, gg, None, None, None, None, None, None, None
 
 

@@ -95,8 +95,18 @@ typedef struct gbdpcg_config {
   uint32_t batch_split; /* batch plans: 0 = one persistent launch for every system;
                            1 = one launch per kernel class present (rows of n <= 14,
                            rows of n = 28, LDS-resident rows), back to back          */
-  uint32_t reserved[4];
+  uint32_t exchange_scope; /* GBDPCG_EXCHANGE_*: how ranks publish their exchange words */
+  uint32_t reserved[3];
 } gbdpcg_config;
+
+/* exchange stores (gbdpcg_config.exchange_scope).  AUTO: workgroup-scope stores,
+ * which complete in the shared L2 of one XCD, when the first exchange of a solve
+ * finds every rank of the system on one XCD, else device-scope stores.  DEVICE:
+ * device-scope stores always (written through to memory; slower, and independent
+ * of the placement).  The environment variable GBDPCG_DEVICE_SCOPE_EXCHANGE=1
+ * forces DEVICE for every call of the process. */
+#define GBDPCG_EXCHANGE_AUTO 0
+#define GBDPCG_EXCHANGE_DEVICE 1
 
 /* kernel selection (gbdpcg_config.kernel) */
 #define GBDPCG_KERNEL_AUTO 0

@@ -67,6 +67,7 @@ struct pcg_config {
   uint32_t kernel = GBDPCG_KERNEL_AUTO;
   uint32_t knots_per_wg = 0;    // 0 = automatic (tuned by gbdpcg_tune_single, else the planner)
   uint32_t threads = 0;         // register kernels: 64 (row broadcast), 256 or 512; 0 = automatic
+  uint32_t exchange_scope = GBDPCG_EXCHANGE_AUTO;  // GBDPCG_EXCHANGE_DEVICE: device-scope stores always
   gbdpcg_result last_result{};  // filled by the device-pointer solve
 
   pcg_config(T exit_tol = pcg_constants::DEFAULT_EPSILON<T>,
@@ -104,6 +105,7 @@ inline gbdpcg_config to_c(const pcg_config<T> *cfg) {
     c.kernel = cfg->kernel;
     c.knots_per_wg = cfg->knots_per_wg;
     c.threads = cfg->threads;
+    c.exchange_scope = cfg->exchange_scope;
   }
   return c;
 }

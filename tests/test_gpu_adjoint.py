@@ -66,6 +66,13 @@ def test_adjoint_against_oracle(gpu, dtype, recurrence, N):
     _, it_a, *_ = O.pcg(S.astype(dtype), P.astype(dtype), glam.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5,
                         173, 1, dtype, fused=recurrence == 1)
     assert abs(res.iters - it_f) <= 1 and abs(res.adj_iters - it_a) <= 1
+    if recurrence == 1:  # and against the reference iteration's own counts (include/pcg.cuh:118-208)
+        _, it_fr, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5,
+                             173, 1, dtype, fused=False)
+        _, it_ar, *_ = O.pcg(S.astype(dtype), P.astype(dtype), glam.astype(dtype), np.zeros(n * N, dtype), n, N,
+                             1e-5, 173, 1, dtype, fused=False)
+        assert abs(res.iters - it_fr) <= 1 and abs(res.adj_iters - it_ar) <= 1, (res.iters, it_fr, res.adj_iters,
+                                                                                  it_ar)
     # g_S is the masked, symmetrised outer product of the returned vectors
     G = -0.5 * (np.outer(gg, lam) + np.outer(lam, gg))
     expect = O.dense_to_cbtd(G, n, N)
@@ -181,3 +188,77 @@ def test_autograd_function(gpu):
     assert rel(tg.grad.cpu().numpy(), gg_d) < 1e-9
     assert rel(tS.grad.cpu().numpy(), gS_d) < 1e-9
     assert torch.isfinite(tS.grad).all()
+
+
+def test_adjoint_fd_n14_N128_fp64(gpu):
+    """The forward + adjoint configuration (n=14, N=128) through its FP64 check
+    path: the device gradient against the exact FP64 gradient (block-LU solves)
+    and against 32 central finite differences (eps 1e-6) of the exact FP64
+    solution map over random symmetric block-tridiagonal directions, plus one
+    direction in gamma."""
+    n, N, dtype = 14, 128, np.float64
+    S, P, g, glam = system(n, N, 4128, dtype)
+    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=1e-13, max_iter=800, exit_mode=1, recurrence=1)
+    assert res.status == 0 and res.adj_status == 0
+    mask = O.pattern_mask(n, N)
+    x = O.block_solve(S, g, n, N)
+    gg_x = O.block_solve(S, glam, n, N)
+    gS_x = O.dense_to_cbtd(-0.5 * (np.outer(gg_x, x) + np.outer(x, gg_x)), n, N) * mask
+    assert rel(gg, gg_x) < 1e-8 and rel(gS, gS_x) < 1e-8, (rel(gg, gg_x), rel(gS, gS_x))
+
+    def L(S_, g_):
+        return float(glam @ O.block_solve(S_, g_, n, N))
+
+    rng = np.random.default_rng(128)
+    eps = 1e-6
+    worst = 0.0
+    for k in range(32):
+        R = rng.normal(size=3 * n * n * N) * mask
+        # symmetric direction in cbtd storage: block (k, k+1) is the transpose of (k+1, k)
+        Dir = O.dense_to_cbtd(0.5 * (O.cbtd_to_dense(R, n, N) + O.cbtd_to_dense(R, n, N).T), n, N)
+        fd = (L(S + eps * Dir, g) - L(S - eps * Dir, g)) / (2 * eps)
+        an = float(gS @ Dir)
+        worst = max(worst, abs(fd - an) / max(1.0, abs(an)))
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (k, fd, an)
+    dgam = rng.normal(size=n * N)
+    fd = (L(S, g + eps * dgam) - L(S, g - eps * dgam)) / (2 * eps)
+    assert abs(fd - float(gg @ dgam)) <= 1e-6 * max(1.0, abs(fd))
+    print(f"n=14 N=128 FP64: iters {res.iters}/{res.adj_iters}, worst FD rel mismatch {worst:.1e}")
+
+
+def test_autograd_function_fp32(gpu):
+    """The autograd wrapper on the FP32 path at n=14 (the forward + adjoint
+    configuration's precision and size class) against the exact FP64 gradient."""
+    import torch
+    from gbdpcg.autograd import pcg_solve
+    n, N = 14, 128
+    S, P, g, glam = system(n, N, 3131, np.float64)
+    tS = _t(S, np.float32).requires_grad_(True)
+    tg = _t(g, np.float32).requires_grad_(True)
+    lam = pcg_solve(tS, tg, _t(P, np.float32), n, N, tol=1e-5, max_iter=173)
+    (lam * _t(glam, np.float32)).sum().backward()
+    x = O.block_solve(S.astype(np.float32).astype(np.float64), g.astype(np.float32).astype(np.float64), n, N)
+    gg_x = O.block_solve(S.astype(np.float32).astype(np.float64), glam.astype(np.float32).astype(np.float64), n, N)
+    gS_x = O.dense_to_cbtd(-0.5 * (np.outer(gg_x, x) + np.outer(x, gg_x)), n, N) * O.pattern_mask(n, N)
+    e_lam = rel(lam.detach().cpu().numpy().astype(np.float64), x)
+    e_gg = rel(tg.grad.cpu().numpy().astype(np.float64), gg_x)
+    e_gS = rel(tS.grad.cpu().numpy().astype(np.float64), gS_x)
+    print(f"autograd FP32 n=14 N=128: lambda {e_lam:.1e}, g_gamma {e_gg:.1e}, g_S {e_gS:.1e}")
+    assert e_lam <= 1e-4 and e_gg <= 1e-4 and e_gS <= 1e-4
+    assert tS.grad.dtype == torch.float32 and torch.isfinite(tS.grad).all()
+
+
+def test_autograd_raises_when_a_solve_does_not_converge(gpu):
+    """An unconverged forward (max_iter too small) or a non-finite adjoint
+    (NaN in the upstream gradient) raises instead of returning a wrong value."""
+    from gbdpcg.autograd import pcg_solve
+    n, N = 6, 12
+    S, P, g, glam = system(n, N, 37, np.float64)
+    tS, tg, tP = _t(S, np.float64).requires_grad_(True), _t(g, np.float64).requires_grad_(True), _t(P, np.float64)
+    with pytest.raises(RuntimeError, match="MAX_ITER"):
+        pcg_solve(tS, tg, tP, n, N, tol=1e-13, max_iter=2)
+    lam = pcg_solve(tS, tg, tP, n, N, tol=1e-12, max_iter=500)
+    bad = glam.copy()
+    bad[3] = np.nan
+    with pytest.raises(RuntimeError, match="NONFINITE"):
+        (lam * _t(bad, np.float64)).sum().backward()

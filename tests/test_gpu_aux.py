@@ -296,10 +296,15 @@ def test_invalid_arguments(gpu):
     n, N = 4, 8
     S, g = lqr(n, N, 1)
     dS, dg, lam = dev(S, np.float32), dev(g, np.float32), dev(np.zeros(n * N), np.float32)
+    import ctypes
     for kw in (dict(n=0, N=N), dict(n=n, N=0)):
-        with pytest.raises(gbdpcg.GbdpcgError) as e:
+        with pytest.raises(ValueError):  # the binding checks sizes before calling the library
             gbdpcg.solve(dS, dg, lam, kw["n"], kw["N"], None)
-        assert e.value.code == gbdpcg.ERR_INVALID
+        # and the C ABI rejects them itself
+        cfg = gbdpcg.make_config()
+        rc = gbdpcg.lib().gbdpcg_solve_device_f32(kw["n"], kw["N"], dS.data_ptr(), None, dg.data_ptr(),
+                                                  lam.data_ptr(), ctypes.byref(cfg), None)
+        assert rc == gbdpcg.ERR_INVALID
     with pytest.raises(gbdpcg.GbdpcgError) as e:
         gbdpcg.solve(dS, dg, lam, n, N, None, max_iter=10 ** 6)
     assert e.value.code == gbdpcg.ERR_INVALID

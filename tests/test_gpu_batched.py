@@ -44,6 +44,10 @@ def check(systems, host, results, dtype, recurrence, adjoint=False):
         _, it_same, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N,
                                d["tol"], 173, 1, dtype, fused=recurrence == 1)
         assert abs(r.iters - it_same) <= 1, (n, N, r.iters, it_same)
+        if recurrence == 1:  # the fused form against the reference iteration's own count as well
+            _, it_refform, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n,
+                                      N, d["tol"], 173, 1, dtype, fused=False)
+            assert abs(r.iters - it_refform) <= 1, (n, N, r.iters, it_refform)
         lam_ref, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 0.0, r.iters)
         assert rel(d["lam"].cpu().numpy(), lam_ref) <= tol, (n, N)
         if adjoint:

@@ -45,15 +45,20 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def check_against_oracle(S, P, g, n, N, dtype, tol, max_iter, mode, recurrence, wgs=0, lam0=None):
+def check_against_oracle(S, P, g, n, N, dtype, tol, max_iter, mode, recurrence, wgs=0, lam0=None, **extra):
     import gbdpcg
     lam0 = np.zeros(n * N, dtype) if lam0 is None else lam0.astype(dtype)
     lam_gpu, res = gpu_solve(S, P, g, lam0, n, N, dtype, tol=tol, max_iter=max_iter, exit_mode=mode,
-                             recurrence=recurrence, wgs=wgs)
+                             recurrence=recurrence, wgs=wgs, **extra)
     fused = recurrence == gbdpcg.RECUR_FUSED
     # same-precision oracle: iteration count
     _, it_same, st_same, *_ = O.pcg(S, P, g, lam0, n, N, tol, max_iter, mode, dtype, fused=fused)
     assert abs(res.iters - it_same) <= 1, (res.iters, it_same)
+    if fused:
+        # the fused form is held to the reference iteration's own count too
+        # (include/pcg.cuh:118-208, same precision)
+        _, it_refform, *_ = O.pcg(S, P, g, lam0, n, N, tol, max_iter, mode, dtype, fused=False)
+        assert abs(res.iters - it_refform) <= 1, ("vs reference form", res.iters, it_refform)
     # FP64 oracle for the same number of iterations: lambda
     lam_ref, it_ref, *_ = O.pcg(S.astype(np.float64), P.astype(np.float64), g.astype(np.float64),
                                 lam0.astype(np.float64), n, N, 0.0, res.iters, mode, np.float64)

@@ -197,3 +197,56 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for field, _ in py._fields_:
             assert got[(cname, field)] == getattr(py, field).offset, (cname, field)
+
+
+def test_binding_rejects_mismatched_tensors_before_launch():
+    """The binding checks dtype, element count and device of every tensor before
+    any launch (a short or mistyped buffer would otherwise be read or written out
+    of bounds on the device).  CPU tensors suffice: the checks run first."""
+    import torch
+    n, N = 6, 8
+    S = torch.zeros(3 * n * n * N, dtype=torch.float64)
+    vec = torch.zeros(n * N, dtype=torch.float64)
+    cases = [
+        dict(gamma=vec.float(), lam=vec),                        # gamma in FP32 with FP64 S
+        dict(gamma=vec, lam=vec[:-1].clone()),                   # short lam
+        dict(gamma=torch.zeros(n * N + 1, dtype=torch.float64), lam=vec),
+        dict(gamma=vec, lam=vec, Pinv=S[:-n].clone()),           # short Pinv
+    ]
+    for kw in cases:
+        with pytest.raises(ValueError):
+            gbdpcg.solve(S, kw["gamma"], kw["lam"], n, N, kw.get("Pinv"))
+        with pytest.raises(ValueError):
+            gbdpcg.solve_async(S, kw["gamma"], kw["lam"], n, N, kw.get("Pinv"))
+    with pytest.raises(ValueError):  # n N mismatch against S
+        gbdpcg.solve(S, vec, vec, n, N + 1)
+    with pytest.raises(ValueError):  # adjoint outputs: short gS, FP32 ggamma
+        gbdpcg.adjoint(S, vec, vec, vec, vec, S[:-1].clone(), n, N, S)
+    with pytest.raises(ValueError):
+        gbdpcg.adjoint(S, vec, vec, vec, vec.float(), S, n, N, S)
+    with pytest.raises(ValueError):  # batch descriptors are checked per system
+        gbdpcg.Batch([dict(S=S, Pinv=S, gamma=vec, lam=vec[:-1].clone(), n=n, N=N)])
+    with pytest.raises(ValueError):
+        gbdpcg.Batch([dict(S=S, Pinv=S, gamma=vec, lam=vec, n=n, N=N, adjoint=True, glam=vec,
+                           ggamma=vec, gS=None)])
+    # host-pointer call: sizes against (n, N) before anything is copied
+    Sh = np.zeros(3 * n * n * N)
+    with pytest.raises(ValueError):
+        gbdpcg.solve_host(Sh, np.zeros(n * N), np.zeros(n * N - 1), n, N)
+    with pytest.raises(ValueError):
+        gbdpcg.solve_host(Sh[:-1].copy(), np.zeros(n * N), np.zeros(n * N), n, N)
+
+
+def test_exchange_scope_is_validated():
+    L = gbdpcg.lib()
+    buf = (ctypes.c_float * 8)()
+    cfg = gbdpcg.make_config(exchange_scope=2)
+    assert L.gbdpcg_solve_f32(buf, None, buf, buf, 2, 3, ctypes.byref(cfg)) == gbdpcg.ERR_INVALID
+    assert "exchange_scope" in gbdpcg.last_error()
+    h = ctypes.c_void_p()
+    d = (gbdpcg.SystemDesc * 1)()
+    assert L.gbdpcg_batch_create(ctypes.byref(h), gbdpcg.F32, d, 1, ctypes.byref(cfg), 0) == gbdpcg.ERR_INVALID
+    # the default is the automatic choice
+    c0 = gbdpcg.Config()
+    L.gbdpcg_config_init(ctypes.byref(c0))
+    assert c0.exchange_scope == gbdpcg.EXCHANGE_AUTO

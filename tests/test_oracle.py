@@ -293,3 +293,39 @@ def test_adjoint_matches_finite_differences():
     assert np.all(gS.reshape(N, 3, n * n)[0, 0] == 0) and np.all(gS.reshape(N, 3, n * n)[-1, 2] == 0)
     G = O.cbtd_to_dense(gS, n, N)
     assert np.allclose(G, G.T)
+
+
+FIXTURES = ["lqr_n2_N3_jacobi.json", "lqr_n14_N32_ss.json", "lqr_n14_N128_ss.json"]
+
+
+@pytest.mark.parametrize("fname", FIXTURES)
+def test_oracle_reproduces_committed_fixtures(fname):
+    """The committed fixtures (tests/golden/make_fixtures.py) hold the oracle's
+    outputs on seeded LQR systems; the oracle must still produce them: FP64
+    iteration counts and statuses exactly, lambda to 1e-12 and the r'z history to
+    1e-9 relative; FP32 counts within one iteration (they depend on rounding)."""
+    import json
+    import gbdpcg
+    d = json.load(open(os.path.join(HERE, "golden", fname)))
+    gen = d["generator"]
+    n, N = gen["n"], gen["N"]
+    S, g = gbdpcg.lqr_generate(n, gen["m"], N, gen["seed"], dtype=np.float64)
+    P = O.precond(S, n, N, gen["precond"])
+    assert abs(np.linalg.norm(S) - d["checks"]["S_fro"]) <= 1e-12 * d["checks"]["S_fro"]
+    assert abs(g.sum() - d["checks"]["gamma_sum"]) <= 1e-9 * max(1.0, abs(d["checks"]["gamma_sum"]))
+    assert abs(np.linalg.norm(P) - d["checks"]["Pinv_fro"]) <= 1e-10 * d["checks"]["Pinv_fro"]
+    for key, want in d["outputs"].items():
+        mode_name, dt_name, form = key.split("_")
+        dt = np.float64 if dt_name == "f64" else np.float32
+        mode = O.EXIT_REL if mode_name == "rel" else O.EXIT_ABS
+        lam, it, st, eta, rr, hist = O.pcg(S.astype(dt), P.astype(dt), g.astype(dt), np.zeros(n * N, dt), n, N,
+                                           d["tol"], d["max_iter"], mode, dt, fused=form == "fused")
+        if dt == np.float64:
+            assert (it, st) == (want["iters"], want["status"]), (key, it, st)
+        else:
+            assert abs(it - want["iters"]) <= 1 and st == want["status"], (key, it, st)
+        if "lambda" in want:
+            ref = np.array(want["lambda"])
+            assert np.linalg.norm(lam - ref) <= 1e-12 * np.linalg.norm(ref), key
+            h = np.array(want["eta_history"])
+            assert len(hist) == len(h) and np.allclose(hist, h, rtol=1e-9, atol=0), key
