@@ -24,7 +24,7 @@ HOST_FLAGS = ["--offload-arch=" + ARCH, "-O2", "-std=c++17", "-fPIC", "-Wall"]
 
 HIP_SOURCES = ["pcg_kernel.hip", "pcg_reg_kernel.hip", "pcg_reg_ref_kernel.hip", "pcg_rowcast_kernel.hip",
                "pcg_pair_kernel_f32.hip", "pcg_pair_kernel_f64.hip",
-               "aux_kernels.hip"]
+               "aux_kernels.hip", "noalloc.hip"]
 CPP_SOURCES = ["api.cpp", "batch.cpp", "problem.cpp", "tune.cpp"]
 
 

@@ -636,6 +636,66 @@ static int solve_csr_impl(int dtype, const uint32_t *Sr, const uint32_t *Sc, con
                           const uint32_t *Pc, const T *Pv, const T *h_gamma, T *h_lambda, uint32_t n, uint32_t N,
                           const gbdpcg_config *cfg);
 
+// A forward solve launched on caller-provided device scratch (no workspace, no
+// allocation, no memset node): `scratch` holds kScratchHeaderWords words (the
+// result record when d_result is null, then the error word) followed by the
+// exchange records.  The error word and the records are cleared by a fill
+// kernel before the solve, so every call starts from zeroed records.  Used by
+// the no-allocation entry point (noalloc.hip).
+int solve_on_scratch(int dtype, uint32_t n, uint32_t N, const void *S, const void *Pinv, const void *gamma,
+                     void *lambda, const gbdpcg_config *cfg, gbdpcg_result *d_result, uint64_t *scratch,
+                     size_t scratch_words) {
+  int rc = check_args(dtype, n, N, S, gamma, lambda, nullptr, nullptr, nullptr, KF_FWD, cfg);
+  if (rc < 0) return rc;
+  if (!Pinv) {
+    set_error("the no-allocation solve needs Pinv (forming it would need device memory)");
+    return GBDPCG_ERR_INVALID;
+  }
+  if (!scratch || scratch_words <= kScratchHeaderWords) {
+    set_error("no scratch buffer");
+    return GBDPCG_ERR_INVALID;
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED;
+  if (fused && N < 2) fused = false;
+  Layout L;
+  rc = choose_layout(dev, dtype, n, N, fused, cfg, &L, false);
+  if (rc < 0) return rc;
+  const size_t words = 2 * (size_t)L.G * rec_words(n, (uint32_t)(tsize(dtype) / 4));
+  if (words > scratch_words - kScratchHeaderWords) {
+    set_error("n=%u N=%u needs %zu words of exchange records on %u workgroups; the no-allocation scratch holds "
+              "%zu", n, N, words, L.G, scratch_words - kScratchHeaderWords);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
+  hipStream_t stream = static_cast<hipStream_t>(cfg->stream);
+  uint64_t *err_word = scratch + kScratchErrWord;
+  // the error word and every record word of this layout start at zero
+  HIPCHK(fill_u64(err_word, kScratchHeaderWords - kScratchErrWord + words, 0, stream));
+  KSys one{};
+  one.S = S;
+  one.Pinv = Pinv;
+  one.gamma = gamma;
+  one.lambda = lambda;
+  one.slots = scratch + kScratchHeaderWords;
+  one.result = d_result ? d_result : reinterpret_cast<gbdpcg_result *>(scratch);
+  one.err = reinterpret_cast<int *>(err_word);
+  one.tol = cfg->exit_tol;
+  one.n = n;
+  one.N = N;
+  one.G = L.G;
+  one.max_iter = cfg->max_iter;
+  one.flags = KF_FWD | (cfg->exit_mode == GBDPCG_EXIT_REL_RESID ? KF_REL : 0u) | (fused ? KF_FUSED : 0u) |
+              exchange_flags(cfg);
+  if (L.reg) {
+    HIPCHK(reg_launch(dtype, (int)n, L.nt, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream,
+                      RegVariant{fused, L.bc, false}));
+  } else {
+    HIPCHK(pcg_launch(dtype, one, nullptr, nullptr, 0, L.stride, L.grid, L.lds, stream));
+  }
+  return GBDPCG_OK;
+}
+
 }  // namespace gbdpcg
 
 using namespace gbdpcg;

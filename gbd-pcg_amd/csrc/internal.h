@@ -190,6 +190,15 @@ namespace gbdpcg {
 // GBDPCG_DEVICE_SCOPE_EXCHANGE is set to a non-empty value other than "0" (read
 // once); else 0.  cfg may be null (the environment alone decides).
 uint32_t exchange_flags(const gbdpcg_config *cfg);
+// api.cpp: a forward solve on caller-provided device scratch, no allocation
+// (see noalloc.hip).  Scratch layout in 64-bit words: [0, 6) the result record
+// when no d_result is given, word kScratchErrWord the error word, records from
+// kScratchHeaderWords (256-byte aligned).
+constexpr size_t kScratchErrWord = 6;
+constexpr size_t kScratchHeaderWords = 32;
+int solve_on_scratch(int dtype, uint32_t n, uint32_t N, const void *S, const void *Pinv, const void *gamma,
+                     void *lambda, const gbdpcg_config *cfg, gbdpcg_result *d_result, uint64_t *scratch,
+                     size_t scratch_words);
 // batch.cpp: properties of a built plan used by the tuner
 bool batch_uses_registers(const gbdpcg_batch *plan);
 int batch_kernel(const gbdpcg_batch *plan);  // GBDPCG_KERNEL_LDS / _REGISTER / _ROWCAST

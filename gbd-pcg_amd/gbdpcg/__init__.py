@@ -85,6 +85,7 @@ class TuneReport(ctypes.Structure):
 
 
 RESULT_BYTES = ctypes.sizeof(Result)
+NOALLOC_SCRATCH_BYTES = 1 << 20  # GBDPCG_NOALLOC_SCRATCH_BYTES
 
 _lib = None
 
@@ -105,6 +106,8 @@ SIGNATURES = [
     ("gbdpcg_solve_device_f64", _i32, [_u32, _u32, _vp, _vp, _vp, _vp, _PC, _PR]),
     ("gbdpcg_solve_device_async_f32", _i32, [_u32, _u32, _vp, _vp, _vp, _vp, _PC, _vp]),
     ("gbdpcg_solve_device_async_f64", _i32, [_u32, _u32, _vp, _vp, _vp, _vp, _PC, _vp]),
+    ("gbdpcg_solve_device_noalloc_f32", _i32, [_u32, _u32, _vp, _vp, _vp, _vp, _PC, _vp]),
+    ("gbdpcg_solve_device_noalloc_f64", _i32, [_u32, _u32, _vp, _vp, _vp, _vp, _PC, _vp]),
     ("gbdpcg_solve_csr_f32", _i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _PC]),
     ("gbdpcg_solve_csr_f64", _i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _PC]),
     ("gbdpcg_csr_to_cbtd", _i32, [_i32, _vp, _vp, _vp, _u32, _u32, _vp]),
@@ -275,6 +278,23 @@ def solve_async(S, gamma, lam, n: int, N: int, Pinv=None, d_result=None, *, tol=
     cfg = make_config(tol, max_iter, exit_mode, recurrence, precond, wgs, _stream_handle(stream), kernel,
                       threads=threads, exchange_scope=exchange_scope)
     f = lib().gbdpcg_solve_device_async_f64 if dt == F64 else lib().gbdpcg_solve_device_async_f32
+    check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
+            _dev(d_result, "d_result")))
+
+
+def solve_noalloc(S, gamma, lam, n: int, N: int, Pinv, d_result=None, *, tol=1e-6, max_iter=25,
+                  exit_mode=EXIT_ABS_ETA, recurrence=RECUR_REFERENCE, wgs=0, stream=None, kernel=KERNEL_AUTO,
+                  threads=0, exchange_scope=EXCHANGE_AUTO) -> None:
+    """solve_async without any device allocation: the exchange records live in the
+    library's fixed per-device scratch (NOALLOC_SCRATCH_BYTES).  Pinv is
+    required.  Calls that overlap on one device race on that scratch."""
+    if Pinv is None:
+        raise ValueError("solve_noalloc needs Pinv")
+    dt = _check_system(S, n, N, mats=(("Pinv", Pinv),), vecs=(("gamma", gamma), ("lam", lam)))
+    _check_result_buffer(d_result)
+    cfg = make_config(tol, max_iter, exit_mode, recurrence, PRECOND_SS, wgs, _stream_handle(stream), kernel,
+                      threads=threads, exchange_scope=exchange_scope)
+    f = lib().gbdpcg_solve_device_noalloc_f64 if dt == F64 else lib().gbdpcg_solve_device_noalloc_f32
     check(f(n, N, _dev(S, "S"), _dev(Pinv, "Pinv"), _dev(gamma, "gamma"), _dev(lam, "lam"), ctypes.byref(cfg),
             _dev(d_result, "d_result")))
 

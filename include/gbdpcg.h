@@ -203,6 +203,28 @@ int gbdpcg_solve_device_async_f64(uint32_t stateSize, uint32_t knotPoints, const
                                   const double *d_Pinv, const double *d_gamma, double *d_lambda,
                                   const gbdpcg_config *cfg, gbdpcg_result *d_result);
 
+/* No-allocation solve (extension; no counterpart in the reference).  For callers
+ * that cannot allocate device memory -- code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc -- the same solve as
+ * gbdpcg_solve_device_async_*, with its device scratch (exchange records, result
+ * record, error word) in a fixed array of GBDPCG_NOALLOC_SCRATCH_BYTES held by
+ * the library on each device for the life of the process: the call allocates
+ * nothing and records no memset (a fill kernel clears the scratch).  d_Pinv is
+ * required (forming it would need device memory).  The result record goes to
+ * d_result when it is not NULL (status GBDPCG_TIMED_OUT there reports a failed
+ * exchange wait).  Systems whose exchange records exceed the scratch are
+ * rejected with GBDPCG_ERR_TOO_LARGE before any launch (n = 14 FP32 up to about
+ * N = 3800; n = 28 FP64 up to about N = 900).  Every call on one device shares
+ * that device's scratch: calls that can overlap on one device race, so keep them
+ * on one stream (or synchronise between them).  Returns 0 after the launch. */
+#define GBDPCG_NOALLOC_SCRATCH_BYTES (1u << 20)
+int gbdpcg_solve_device_noalloc_f32(uint32_t stateSize, uint32_t knotPoints, const float *d_S,
+                                    const float *d_Pinv, const float *d_gamma, float *d_lambda,
+                                    const gbdpcg_config *cfg, gbdpcg_result *d_result);
+int gbdpcg_solve_device_noalloc_f64(uint32_t stateSize, uint32_t knotPoints, const double *d_S,
+                                    const double *d_Pinv, const double *d_gamma, double *d_lambda,
+                                    const gbdpcg_config *cfg, gbdpcg_result *d_result);
+
 /* CSR input: the reference stubs this overload (include/interface.cuh:8-20 prints
  * "NOT IMPLEMENTED" and exits with 12).  Here the CSR matrices (host pointers,
  * 0-based, square nN x nN) are converted to cbtd on the host and solved; entries
