@@ -212,7 +212,10 @@ def main():
     # once, gamma and lambda0 read, lambda written)
     ts = 4
     hbm_bytes = (2 * 3 * n * n * N + 3 * n * N) * ts
-    achieved = hbm_bytes / (kern_us * 1e-6) / 1e9
+    # the kernel's average launch duration: HIP events over the timed region of
+    # back-to-back launches on the launch stream, over the launches (per-launch
+    # event pairs, kernel_us, add the events' own overhead)
+    achieved = hbm_bytes / (us_per_solve * 1e-6) / 1e9
     # measured HBM bytes per launch: rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this
     # same command, scaled by the counters' reading of a plain copy of known size
     # (the counters are looked up under the name of the kernel this layout runs)
