@@ -125,6 +125,24 @@ struct SlotBuf {
   bool dirty = false;  // epochs inconsistent (after a timed-out wait): re-zero before use
 };
 
+// Layout of one system's launch
+struct Layout {
+  uint32_t G = 0, stride = 1, grid = 0;
+  size_t lds = 0;
+  bool reg = false;    // register-resident kernel
+  bool bc = false;     // ... in the row-broadcast layout
+  bool pair = false;   // ... running forward and adjoint together
+  bool fused = true;   // its recurrence (the LDS kernel takes it from the flags)
+  int nt = 256;        // threads per workgroup
+};
+
+// A layout chosen for (dtype, n, N, recurrence, pairing, the config fields that
+// steer the choice), valid while the tuner's choices are unchanged (gen).
+struct LayoutMemo {
+  uint64_t k0, k1, k2, gen;
+  Layout L;
+};
+
 struct Workspace {
   std::mutex mu;  // held by a call from its first enqueue to its last read of the fields below
   int device = 0;
@@ -142,6 +160,8 @@ struct Workspace {
   // lambda, read and written by the kernels in place
   void *h_stage = nullptr;
   size_t stage_bytes = 0;
+  // layouts already chosen for this workspace's calls (see cached_layout)
+  std::vector<LayoutMemo> layouts;
 };
 
 static std::mutex g_ws_mu;
@@ -231,17 +251,9 @@ static int grow_pinned(Workspace *ws, size_t need) {
 // ---------------------------------------------------------------------------
 // workgroup count for a single system
 // ---------------------------------------------------------------------------
-struct Layout {
-  uint32_t G = 0, stride = 1, grid = 0;
-  size_t lds = 0;
-  bool reg = false;    // register-resident kernel
-  bool bc = false;     // ... in the row-broadcast layout
-  bool pair = false;   // ... running forward and adjoint together
-  bool fused = true;   // its recurrence (the LDS kernel takes it from the flags)
-  int nt = 256;        // threads per workgroup
-};
 
 static uint32_t ceil_div(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
 
 // Latency-oriented choice: about 8 knots per workgroup (the per-iteration cost is
 // dominated by the two cross-workgroup exchanges, whose latency grows slowly with
@@ -424,6 +436,36 @@ static int choose_layout(int dev, int dtype, uint32_t n, uint32_t N, bool fused,
   return rc;
 }
 
+// choose_layout for a workspace's calls, remembered per workspace: a repeated
+// solve of the same shape and config skips the tuner-cache key, the device
+// queries and the occupancy check (microseconds on every call otherwise).  The
+// caller holds ws->mu.
+static int cached_layout(Workspace *ws, int dtype, uint32_t n, uint32_t N, bool fused, const gbdpcg_config *cfg,
+                         Layout *L, bool pair) {
+  const uint64_t k0 = (uint64_t)n | ((uint64_t)N << 32);
+  const uint64_t k1 = (uint64_t)cfg->grid[0] | ((uint64_t)cfg->knots_per_wg << 32);
+  const uint64_t k2 = (uint64_t)(dtype & 3) | ((uint64_t)fused << 2) | ((uint64_t)pair << 3) |
+                      ((uint64_t)(cfg->exit_mode & 3) << 4) | ((uint64_t)(cfg->kernel & 15) << 8) |
+                      ((uint64_t)cfg->threads << 16);
+  const uint64_t gen = tune_generation();
+  for (const LayoutMemo &m : ws->layouts)
+    if (m.k0 == k0 && m.k1 == k1 && m.k2 == k2 && m.gen == gen) {
+      *L = m.L;
+      return GBDPCG_OK;
+    }
+  const int rc = choose_layout(ws->device, dtype, n, N, fused, cfg, L, pair);
+  if (rc < 0) return rc;
+  for (LayoutMemo &m : ws->layouts)
+    if (m.k0 == k0 && m.k1 == k1 && m.k2 == k2) {
+      m.gen = gen;
+      m.L = *L;
+      return GBDPCG_OK;
+    }
+  if (ws->layouts.size() >= 32) ws->layouts.erase(ws->layouts.begin());
+  ws->layouts.push_back(LayoutMemo{k0, k1, k2, gen, *L});
+  return GBDPCG_OK;
+}
+
 static int validate_cfg(const gbdpcg_config *cfg) {
   if (!cfg) {
     set_error("null config");
@@ -516,7 +558,7 @@ static int solve_locked(Workspace *ws, int dtype, uint32_t n, uint32_t N, const 
   // forward + adjoint on a register kernel (n <= 14): both solves in one loop
   const bool pair = fused && (flags & KF_FWD) && (flags & KF_ADJ) && n <= 14;
   Layout L;
-  int rc = choose_layout(ws->device, dtype, n, N, fused, cfg, &L, pair);
+  int rc = cached_layout(ws, dtype, n, N, fused, cfg, &L, pair);
   if (rc < 0) return rc;
 
   if (Pinv == nullptr) {

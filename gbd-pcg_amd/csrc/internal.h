@@ -182,6 +182,7 @@ std::string device_tag(int dev);
 bool tune_lookup(const std::string &key, TuneChoice *out);
 void tune_store(const std::string &key, const TuneChoice &c);
 void tune_forget_all();
+uint64_t tune_generation();  // changes whenever the cached choices change
 }  // namespace gbdpcg
 struct gbdpcg_batch;
 namespace gbdpcg {

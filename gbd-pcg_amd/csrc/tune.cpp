@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -49,13 +50,25 @@ void load_file_locked() {
 }  // namespace
 
 std::string device_tag(int dev) {
+  // per device, once: reading the device properties costs microseconds and the
+  // tag is needed on every automatic single solve
+  static std::mutex mu;
+  static std::map<int, std::string> tags;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = tags.find(dev);
+  if (it != tags.end()) return it->second;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
   std::string arch = prop.gcnArchName;
   for (char &ch : arch)
     if (ch == ' ' || ch == ':') ch = '-';
-  return arch + "-cu" + std::to_string(prop.multiProcessorCount);
+  return tags[dev] = arch + "-cu" + std::to_string(prop.multiProcessorCount);
 }
+
+// Bumped whenever a choice is stored or the choices are dropped, so that callers
+// holding a layout derived from the cache know to look again.
+static std::atomic<uint64_t> g_tune_gen{1};
+uint64_t tune_generation() { return g_tune_gen.load(std::memory_order_acquire); }
 
 bool tune_lookup(const std::string &key, TuneChoice *out) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -70,6 +83,7 @@ void tune_store(const std::string &key, const TuneChoice &c) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   load_file_locked();
   g_tune[key] = c;
+  g_tune_gen.fetch_add(1, std::memory_order_acq_rel);
   const char *path = cache_path();
   if (!path) return;
   FILE *f = fopen(path, "a");
@@ -82,6 +96,7 @@ void tune_forget_all() {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune.clear();
   g_file_loaded = true;  // the file (if any) is not re-read
+  g_tune_gen.fetch_add(1, std::memory_order_acq_rel);
 }
 
 }  // namespace gbdpcg
