@@ -462,6 +462,34 @@ __device__ __forceinline__ T chunk_dot(const T (&mat)[CW], const T *__restrict__
   return (a0 + a1) + (a2 + a3);
 }
 
+// The paired kernels' FP32 instances held to 128 VGPRs keep their Pinv chunks in
+// LDS rather than in registers (see pair_pinv_lds): float4 block b of lane t's
+// chunk sits at float4 index b * NT + t, so a wave's 16-byte reads of one block
+// are contiguous and conflict-free.  Two dot products (the two sides of a
+// paired solve) share each matrix read; each side's four accumulator chains
+// follow chunk_dot's order, so the bits are those of chunk_dot.
+template <int CW, int NT>
+__device__ __forceinline__ void chunk_dot2_lds(const float *__restrict__ plds, int tid, const float *__restrict__ xa,
+                                               const float *__restrict__ xb, float &ya, float &yb) {
+  float a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+#pragma unroll
+  for (int j = 0; j < CW; j += 4) {
+    const float4 mv = *reinterpret_cast<const float4 *>(plds + ((size_t)(j / 4) * NT + tid) * 4);
+    const float4 u = *reinterpret_cast<const float4 *>(xa + j);
+    const float4 v = *reinterpret_cast<const float4 *>(xb + j);
+    a0 = __builtin_fmaf(mv.x, u.x, a0);
+    a1 = __builtin_fmaf(mv.y, u.y, a1);
+    a2 = __builtin_fmaf(mv.z, u.z, a2);
+    a3 = __builtin_fmaf(mv.w, u.w, a3);
+    b0 = __builtin_fmaf(mv.x, v.x, b0);
+    b1 = __builtin_fmaf(mv.y, v.y, b1);
+    b2 = __builtin_fmaf(mv.z, v.z, b2);
+    b3 = __builtin_fmaf(mv.w, v.w, b3);
+  }
+  ya = (a0 + a1) + (a2 + a3);
+  yb = (b0 + b1) + (b2 + b3);
+}
+
 // Sum over the LPR lanes sharing one matrix row; all of them get the same bits.
 template <int LPR, typename T>
 __device__ __forceinline__ T group_sum(T v) {
@@ -682,6 +710,35 @@ __device__ __forceinline__ T row_mv(const RegItem<T, NS, NT, BC> &it, const T (&
   } else {
     T y = act ? chunk_dot<T, CW>(mat, vec + it.m * R::nP + it.c * CW) : (T)0;
     return group_sum<R::LPR>(y);
+  }
+}
+
+// Whether a paired instance keeps its Pinv chunks in LDS: the FP32 column-chunk
+// instances of n <= 14 at 512 threads, held to 128 VGPRs so that two workgroups
+// share a CU (with the chunks in registers they spill ~100 B per lane).  The
+// host sizes LDS with the same rule (reg_lds_bytes).
+__host__ __device__ constexpr bool pair_pinv_lds(size_t tsize, int nt, bool bc, int ns) {
+  return tsize == 4 && nt == 512 && !bc && ns <= 14;
+}
+
+// u = Pinv r of both sides of a paired solve: from the register chunk, or from
+// the LDS chunk (PLDS).
+template <bool PLDS, typename T, int NS, int NT, bool BC, int CWR>
+__device__ __forceinline__ void pinv_mv2(const RegItem<T, NS, NT, BC> &it, const T (&mM)[CWR], const T *plds,
+                                         const T *va, const T *vb, int tid, T &ya, T &yb) {
+  using R = RegItem<T, NS, NT, BC>;
+  if constexpr (PLDS) {
+    static_assert(sizeof(T) == 4 && !BC, "LDS-resident Pinv: FP32 column chunks only");
+    T a = 0, b = 0;
+    if (it.act_u) {
+      const int off = it.m * R::nP + it.c * R::CW;
+      chunk_dot2_lds<R::CW, NT>(plds, tid, va + off, vb + off, a, b);
+    }
+    ya = group_sum<R::LPR>(a);
+    yb = group_sum<R::LPR>(b);
+  } else {
+    ya = row_mv(it, mM, va, it.act_u);
+    yb = row_mv(it, mM, vb, it.act_u);
   }
 }
 
@@ -1132,6 +1189,14 @@ template <typename T, int NS, int NT, bool BC>
 __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *lds) {
   using R = RegItem<T, NS, NT, BC>;
   constexpr int n = R::n, nP = R::nP, CW = R::CW;
+  // Pinv chunks in LDS (the first NT * CW values) for the instances held to
+  // 128 VGPRs; the vectors follow
+  constexpr bool PLDS = pair_pinv_lds(sizeof(T), NT, BC, NS);
+  T *plds = nullptr;
+  if constexpr (PLDS) {
+    plds = lds;
+    lds += (size_t)NT * CW;
+  }
   R it;
   it.N = (int)d.N;
   it.G = (int)d.G;
@@ -1194,12 +1259,22 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
   }
   for (int i = opaque_tid(); i < total; i += NT) lds[i] = (T)0;
 
-  T mM[CW], mS[CW];
+  T mM[PLDS ? 1 : CW], mS[CW];
   if constexpr (BC) {
     load_row<T, NS, true>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
     load_row<T, NS, true>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
-  } else {
+  } else if constexpr (!PLDS) {
     load_chunk<T, NS, CW, kLoadGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW, kLoadGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+  } else {
+    // this lane's Pinv chunk through registers into its LDS slots (read back by
+    // this lane only: no barrier needed between the store and the reads)
+    T tmp[CW];
+    load_chunk<T, NS, CW, kLoadGuarded>(tmp, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+#pragma unroll
+    for (int j = 0; j < CW; j += 4)
+      *reinterpret_cast<float4 *>(plds + ((size_t)(j / 4) * NT + t) * 4) = make_float4(tmp[j], tmp[j + 1], tmp[j + 2],
+                                                                                        tmp[j + 3]);
     load_chunk<T, NS, CW, kLoadGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
   }
   __syncthreads();
@@ -1252,8 +1327,12 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
     rgA = it.ghost ? rA[it.gpos] : (T)0;
     rgB = it.ghost ? rB[it.gpos] : (T)0;
     // first u = Pinv r, w = S u of both sides
-    uA_i = row_mv(it, mM, rA, it.act_u);
-    uB_i = row_mv(it, mM, rB, it.act_u);
+    if constexpr (PLDS) {
+      pinv_mv2<PLDS>(it, mM, plds, rA, rB, t, uA_i, uB_i);
+    } else {
+      uA_i = row_mv(it, mM, rA, it.act_u);
+      uB_i = row_mv(it, mM, rB, it.act_u);
+    }
     if (it.act_u && c == 0) {
       uA[(m + 1) * nP + row] = uA_i;
       uB[(m + 1) * nP + row] = uB_i;
@@ -1371,8 +1450,12 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
       rB[it.gpos] = rgB;
     }
     __syncthreads();
-    uA_i = row_mv(it, mM, rA, it.act_u);
-    uB_i = row_mv(it, mM, rB, it.act_u);
+    if constexpr (PLDS) {
+      pinv_mv2<PLDS>(it, mM, plds, rA, rB, t, uA_i, uB_i);
+    } else {
+      uA_i = row_mv(it, mM, rA, it.act_u);
+      uB_i = row_mv(it, mM, rB, it.act_u);
+    }
     if (it.act_u && c == 0) {
       uA[(m + 1) * nP + row] = uA_i;
       uB[(m + 1) * nP + row] = uB_i;

@@ -58,7 +58,9 @@ size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt, RegVariant v) {
   const size_t ts = dtype == GBDPCG_F64 ? 8 : 4;
   const uint32_t lpr = reg_lpr(n, nt, (int)ts);
   const size_t VL = (size_t)(K + 4) * padded_n(n) + (v.bc ? 16u : lpr * reg_chunk(n, lpr));
-  if (v.pair) return (8 * VL + 6 * (size_t)(nt / 64) + 8 + 4) * ts;
+  // paired instances that keep their Pinv chunks in LDS: nt chunks ahead of the vectors
+  const size_t pinv = v.pair && pair_pinv_lds(ts, nt, v.bc, (int)n) ? (size_t)nt * reg_chunk(n, lpr) : 0;
+  if (v.pair) return (pinv + 8 * VL + 6 * (size_t)(nt / 64) + 8 + 4) * ts;
   return (5 * VL + 3 * (size_t)(nt / 64) + 4 + 4) * ts;
 }
 
