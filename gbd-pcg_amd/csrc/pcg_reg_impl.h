@@ -846,12 +846,14 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   // fewer live masks leave their batches faster
   constexpr bool kGuarded = !(BF && NS <= 14);
   constexpr int kMode = kGuarded ? kLoadGuarded : (sizeof(T) == 4 && NT == 256) ? kLoadTables : kLoadArith;
+  // S first: loads complete in issue order, so the first product r = b - S x
+  // (and the first exchange) can start while the Pinv chunks are in flight
   if constexpr (BC) {
-    load_row<T, NS, kGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
     load_row<T, NS, kGuarded>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
+    load_row<T, NS, kGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
   } else {
-    load_chunk<T, NS, CW, kMode>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
     load_chunk<T, NS, CW, kMode>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+    load_chunk<T, NS, CW, kMode>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
   }
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
