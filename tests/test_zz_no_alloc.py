@@ -96,3 +96,39 @@ def test_noalloc_rejects_what_does_not_fit(gpu):
                          recurrence=gbdpcg.RECUR_FUSED)
     torch.cuda.synchronize()
     assert _record(rec).status == gbdpcg.CONVERGED
+
+
+def test_noalloc_captured_as_first_call(gpu):
+    """The entry's callers include code under HIP graph capture: captured as the
+    first call for its shape (nothing may allocate inside capture), replayed
+    three times with eager work in between, every replay equals the eager async
+    solve bit for bit, result record included."""
+    import torch
+    import gbdpcg
+    n, N = 14, 72  # a shape no other test solves, so nothing was planned for it before
+    S, P, g = _system(n, N, 31, np.float32)
+    kw = dict(tol=1e-5, max_iter=173, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+    stream = torch.cuda.Stream()
+    lam = torch.zeros(n * N, device="cuda")
+    rec = torch.zeros(gbdpcg.RESULT_BYTES, dtype=torch.uint8, device="cuda")
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=stream):
+        lam.zero_()
+        gbdpcg.solve_noalloc(S, g, lam, n, N, P, rec, stream=stream, **kw)
+    lam_ref = torch.zeros(n * N, device="cuda")
+    rec_ref = torch.zeros(gbdpcg.RESULT_BYTES, dtype=torch.uint8, device="cuda")
+    gbdpcg.solve_async(S, g, lam_ref, n, N, P, rec_ref, **kw)
+    torch.cuda.synchronize()
+    ref = _record(rec_ref)
+    assert ref.status == gbdpcg.CONVERGED
+    for _ in range(3):
+        junk = torch.randn(1 << 16, device="cuda")
+        junk.mul_(2.0)
+        lam.fill_(7.0)
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        r = _record(rec)
+        assert (r.iters, r.status, r.eta, r.rr) == (ref.iters, ref.status, ref.eta, ref.rr)
+        assert torch.equal(lam, lam_ref)
