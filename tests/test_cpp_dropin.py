@@ -51,7 +51,7 @@ def test_facade_on_device(gpu):
     assert rc == 0, out
     r = parse(out)
     d = json.load(open(os.path.join(ROOT, "tests", "golden", "readme_example.json")))
-    exact = np.array(d["lambda_closed_form_survey"])
+    exact = np.array(d["lambda_closed_form"])
     it64 = int(r["readme_fp64"].split()[0].split("=")[1])
     lam64 = np.array([float(v) for v in r["readme_fp64"].split("lambda=")[1].split()])
     lam32 = np.array([float(v) for v in r["readme_fp32"].split("lambda=")[1].split()])

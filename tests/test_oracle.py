@@ -183,7 +183,7 @@ def test_readme_example_fp64():
     exact = np.linalg.solve(O.cbtd_to_dense(S, 2, 3), g)
     assert st == O.CONVERGED and it == 6  # converges in the dimension of the system
     assert np.allclose(lam, exact, rtol=1e-10)
-    assert np.allclose(exact, d["lambda_closed_form_survey"], rtol=1e-8)
+    assert np.allclose(exact, d["lambda_closed_form"], rtol=1e-8)
 
 
 # ---------------------------------------------------------------------------
@@ -227,7 +227,7 @@ def test_block_solve_lqr_system():
 
 def test_block_solve_negative_definite():
     d, S, _, g = readme()
-    assert np.allclose(O.block_solve(S, g, 2, 3), d["lambda_closed_form_survey"], rtol=1e-8)
+    assert np.allclose(O.block_solve(S, g, 2, 3), d["lambda_closed_form"], rtol=1e-8)
 
 
 def test_converged_pcg_matches_direct():
