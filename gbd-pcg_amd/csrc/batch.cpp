@@ -127,7 +127,18 @@ double system_cost(const gbdpcg_system_desc &d, uint32_t G, bool fused, size_t t
 
 // Paired instances for rows of n = 28 (one lane per row): FP32 at 512 threads,
 // FP64 at 256.
-bool pair28(int dtype, int nt) { return nt == (dtype == GBDPCG_F64 ? 256 : 512); }
+// Paired instances for rows of n = 28: FP32 at 512 threads; FP64 at 256 (one
+// lane per row) and, for launches whose systems are all forward + adjoint, at
+// 512 (S in LDS; that instance runs paired items only).
+bool pair28(int dtype, int nt, bool all_pair) {
+  return dtype == GBDPCG_F64 ? (nt == 256 || (nt == 512 && all_pair)) : nt == 512;
+}
+
+bool all_paired(const gbdpcg_system_desc *systems, const std::vector<uint32_t> &idx) {
+  for (uint32_t i : idx)
+    if (!((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT))) return false;
+  return true;
+}
 
 // Plans one launch over the systems `idx` (all of one kernel class): workgroups
 // per system (G, fused), the launch's LDS size, co-resident grid and a gang list
@@ -372,15 +383,18 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
       if (systems[i].n > 14) has_wide = true;
     // runtime-n instances: 0 = every specialised size, -1 = sizes up to 14
     L.kn = one_n ? (int)systems[small[0]].n : has_wide ? 0 : -1;
-    // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads only
-    const bool only256 = dtype == GBDPCG_F64 && (bc || has_wide);
+    // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads, but
+    // for a launch of forward + adjoint systems of n = 28 only, 512 (S in LDS)
+    const bool pair_only = all_paired(systems, small);
+    const bool f64_pair28 = dtype == GBDPCG_F64 && one_n && systems[small[0]].n == 28 && pair_only && !bc;
+    const bool only256 = dtype == GBDPCG_F64 && (bc || has_wide) && !f64_pair28;
     L.nt = (one_n && (want256 || only256)) ? 256 : 512;
     // forward + adjoint systems run both solves in one loop (the paired kernel;
     // other systems of the launch run one pass after the other on it): rows of
     // n <= 14, and rows of n = 28 on their one-lane-per-row instances
     for (uint32_t i : small)
       if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
-    if (has_wide && !(one_n && pair28(dtype, L.nt))) L.pair = false;
+    if (has_wide && !(one_n && pair28(dtype, L.nt, pair_only))) L.pair = false;
     launches.push_back(L);
     members.push_back(small);
   }
@@ -388,10 +402,11 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     BatchLaunch L;
     L.reg = true;
     L.kn = 28;
-    L.nt = (want256 || dtype == GBDPCG_F64) ? 256 : 512;
+    const bool pair_only = all_paired(systems, wide);
+    L.nt = (want256 || (dtype == GBDPCG_F64 && !pair_only)) ? 256 : 512;
     for (uint32_t i : wide)
       if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;
-    L.pair = L.pair && pair28(dtype, L.nt);
+    L.pair = L.pair && pair28(dtype, L.nt, pair_only);
     launches.push_back(L);
     members.push_back(wide);
   }

@@ -1,7 +1,7 @@
 // Paired forward + adjoint register-resident kernel, double instances (the kernel
 // is in pcg_reg_impl.h): both register layouts, stateSize 2, 6, 12, 14 (and
 // mixed sizes at 512 threads), 256 and 512 threads (whole rows: 256 only);
-// stateSize 28 at 256.
+// stateSize 28 at 256 (one lane per row) and 512 (S in LDS, paired items only).
 #include "pcg_reg_impl.h"
 
 namespace gbdpcg {
@@ -35,9 +35,8 @@ const void *pp_n(int n, bool bc) {
     case 6: return pp<6, NT, false>();
     case 12: return pp<12, NT, false>();
     case 14: return pp<14, NT, false>();
-    case 28:  // one lane per row: 256 threads only
-      if constexpr (NT == 256) return pp<28, NT, false>();
-      return nullptr;
+    case 28:  // one lane per row at 256 threads; S in LDS at 512 (paired items only)
+      return pp<28, NT, false>();
     default: return nullptr;
   }
 }

@@ -490,6 +490,54 @@ __device__ __forceinline__ void chunk_dot2_lds(const float *__restrict__ plds, i
   yb = (b0 + b1) + (b2 + b3);
 }
 
+// FP64 chunks held in LDS as 16-byte blocks: block b of slot i at double2 index
+// b * SL + i (a wave's reads of one block are contiguous).  Two dot products
+// (the two sides of a paired solve) share each matrix read, each in chunk_dot's
+// order, so the bits are those of chunk_dot; chunk_dot_lds_d is one of them.
+template <int CW, int SL>
+__device__ __forceinline__ void chunk_dot2_lds_d(const double *__restrict__ mlds, int idx,
+                                                 const double *__restrict__ xa, const double *__restrict__ xb,
+                                                 double &ya, double &yb) {
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0, b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+#pragma unroll
+  for (int j = 0; j < CW; j += 4) {
+    const double2 m0 = *reinterpret_cast<const double2 *>(mlds + ((size_t)(j / 2) * SL + idx) * 2);
+    const double2 m1 = *reinterpret_cast<const double2 *>(mlds + ((size_t)(j / 2 + 1) * SL + idx) * 2);
+    const double2 u0 = *reinterpret_cast<const double2 *>(xa + j);
+    const double2 u1 = *reinterpret_cast<const double2 *>(xa + j + 2);
+    const double2 v0 = *reinterpret_cast<const double2 *>(xb + j);
+    const double2 v1 = *reinterpret_cast<const double2 *>(xb + j + 2);
+    a0 = __builtin_fma(m0.x, u0.x, a0);
+    a1 = __builtin_fma(m0.y, u0.y, a1);
+    a2 = __builtin_fma(m1.x, u1.x, a2);
+    a3 = __builtin_fma(m1.y, u1.y, a3);
+    b0 = __builtin_fma(m0.x, v0.x, b0);
+    b1 = __builtin_fma(m0.y, v0.y, b1);
+    b2 = __builtin_fma(m1.x, v1.x, b2);
+    b3 = __builtin_fma(m1.y, v1.y, b3);
+  }
+  ya = (a0 + a1) + (a2 + a3);
+  yb = (b0 + b1) + (b2 + b3);
+}
+
+template <int CW, int SL>
+__device__ __forceinline__ double chunk_dot_lds_d(const double *__restrict__ mlds, int idx,
+                                                  const double *__restrict__ x) {
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll
+  for (int j = 0; j < CW; j += 4) {
+    const double2 m0 = *reinterpret_cast<const double2 *>(mlds + ((size_t)(j / 2) * SL + idx) * 2);
+    const double2 m1 = *reinterpret_cast<const double2 *>(mlds + ((size_t)(j / 2 + 1) * SL + idx) * 2);
+    const double2 u0 = *reinterpret_cast<const double2 *>(x + j);
+    const double2 u1 = *reinterpret_cast<const double2 *>(x + j + 2);
+    a0 = __builtin_fma(m0.x, u0.x, a0);
+    a1 = __builtin_fma(m0.y, u0.y, a1);
+    a2 = __builtin_fma(m1.x, u1.x, a2);
+    a3 = __builtin_fma(m1.y, u1.y, a3);
+  }
+  return (a0 + a1) + (a2 + a3);
+}
+
 // Sum over the LPR lanes sharing one matrix row; all of them get the same bits.
 template <int LPR, typename T>
 __device__ __forceinline__ T group_sum(T v) {
@@ -721,6 +769,17 @@ __host__ __device__ constexpr bool pair_pinv_lds(size_t tsize, int nt, bool bc, 
   return tsize == 4 && nt == 512 && !bc && ns <= 14;
 }
 
+// Whether a paired instance keeps its S chunks in LDS: FP64 rows of n = 28 at
+// 512 threads (two lanes per row, two waves per SIMD; with both matrices in
+// registers a lane would need ~290 registers).  Slots for the owned rows only:
+// pair_s_slots(nt, n) lanes (two ghost knots' lanes fewer than the workgroup).
+// Such an instance runs paired items only (plans of forward + adjoint systems).
+// The host sizes LDS with the same rules (reg_lds_bytes).
+__host__ __device__ constexpr bool pair_s_lds(size_t tsize, int nt, bool bc, int ns) {
+  return tsize == 8 && nt == 512 && !bc && ns == 28;
+}
+__host__ __device__ constexpr int pair_s_slots(int nt, int ns) { return nt - 2 * 2 * ns; }
+
 // u = Pinv r of both sides of a paired solve: from the register chunk, or from
 // the LDS chunk (PLDS).
 template <bool PLDS, typename T, int NS, int NT, bool BC, int CWR>
@@ -739,6 +798,40 @@ __device__ __forceinline__ void pinv_mv2(const RegItem<T, NS, NT, BC> &it, const
   } else {
     ya = row_mv(it, mM, va, it.act_u);
     yb = row_mv(it, mM, vb, it.act_u);
+  }
+}
+
+// S times one or two vectors in reg_item_pair: from the register chunk, or
+// from the LDS chunk (SLDS: slot = thread index less the left ghost knot's
+// lanes).
+template <bool SLDS, int SL, typename T, int NS, int NT, bool BC, int CWR>
+__device__ __forceinline__ T s_mv1(const RegItem<T, NS, NT, BC> &it, const T (&mS)[CWR], const T *slds, const T *v,
+                                   int tid) {
+  using R = RegItem<T, NS, NT, BC>;
+  if constexpr (SLDS) {
+    T a = 0;
+    if (it.act_w) a = chunk_dot_lds_d<R::CW, SL>(slds, tid - R::LPR * NS, v + it.m * R::nP + it.c * R::CW);
+    return group_sum<R::LPR>(a);
+  } else {
+    return row_mv(it, mS, v, it.act_w);
+  }
+}
+
+template <bool SLDS, int SL, typename T, int NS, int NT, bool BC, int CWR>
+__device__ __forceinline__ void s_mv2(const RegItem<T, NS, NT, BC> &it, const T (&mS)[CWR], const T *slds,
+                                      const T *va, const T *vb, int tid, T &ya, T &yb) {
+  using R = RegItem<T, NS, NT, BC>;
+  if constexpr (SLDS) {
+    T a = 0, b = 0;
+    if (it.act_w) {
+      const int off = it.m * R::nP + it.c * R::CW;
+      chunk_dot2_lds_d<R::CW, SL>(slds, tid - R::LPR * NS, va + off, vb + off, a, b);
+    }
+    ya = group_sum<R::LPR>(a);
+    yb = group_sum<R::LPR>(b);
+  } else {
+    ya = row_mv(it, mS, va, it.act_w);
+    yb = row_mv(it, mS, vb, it.act_w);
   }
 }
 
@@ -1199,6 +1292,15 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
     plds = lds;
     lds += (size_t)NT * CW;
   }
+  // S chunks in LDS (the first pair_s_slots * CW values) for the FP64 n = 28
+  // instance at 512 threads; its vectors need only 8 values of read-ahead
+  constexpr bool SLDS = pair_s_lds(sizeof(T), NT, BC, NS);
+  constexpr int SL = SLDS ? pair_s_slots(NT, NS) : 1;
+  T *slds = nullptr;
+  if constexpr (SLDS) {
+    slds = lds;
+    lds += (size_t)SL * CW;
+  }
   R it;
   it.N = (int)d.N;
   it.G = (int)d.G;
@@ -1207,7 +1309,7 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
   it.b = (int)knot_begin(rank + 1, d.G, d.N);
   it.K = it.b - it.a;
   const int K = it.K;
-  it.VL = (K + 4) * nP + (BC ? 16 : R::LPR * CW);
+  it.VL = (K + 4) * nP + (BC ? 16 : SLDS ? 8 : R::LPR * CW);
   it.slots = d.slots;
   it.tol = d.tol;
   it.rel = (d.flags & KF_REL) != 0;
@@ -1261,10 +1363,24 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
   }
   for (int i = opaque_tid(); i < total; i += NT) lds[i] = (T)0;
 
-  T mM[PLDS ? 1 : CW], mS[CW];
+  T mM[PLDS ? 1 : CW], mS[SLDS ? 1 : CW];
   if constexpr (BC) {
     load_row<T, NS, true>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
     load_row<T, NS, true>(mS, static_cast<const T *>(d.S), kg, row, it.N, it.act_w);
+  } else if constexpr (SLDS) {
+    // this lane's S chunk through registers into its LDS slot (read back by this
+    // lane only), then its Pinv chunk into registers
+    {
+      T tmp[CW];
+      load_chunk<T, NS, CW, kLoadGuarded>(tmp, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+      if (it.act_w) {
+        const int si = t - R::LPR * n;
+#pragma unroll
+        for (int j = 0; j < CW; j += 2)
+          *reinterpret_cast<double2 *>(slds + ((size_t)(j / 2) * SL + si) * 2) = make_double2(tmp[j], tmp[j + 1]);
+      }
+    }
+    load_chunk<T, NS, CW, kLoadGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
   } else if constexpr (!PLDS) {
     load_chunk<T, NS, CW, kLoadGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
     load_chunk<T, NS, CW, kLoadGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
@@ -1301,7 +1417,11 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
     bB = gptr(static_cast<const T *>(d.glambda))[(size_t)kg * n + row];
   }
   // r = rhs - S x (B starts from x = 0: r = g_lambda)
-  T rA_i = bA - row_mv(it, mS, it.xv, it.act_w);
+  T rA_i;
+  if constexpr (SLDS)
+    rA_i = bA - s_mv1<SLDS, SL>(it, mS, slds, it.xv, t);
+  else
+    rA_i = bA - row_mv(it, mS, it.xv, it.act_w);
   T rB_i = bB;
   if (it.owner) {
     rA[own_pos] = rA_i;
@@ -1340,8 +1460,12 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
       uB[(m + 1) * nP + row] = uB_i;
     }
     __syncthreads();
-    wA_i = row_mv(it, mS, uA, it.act_w);
-    wB_i = row_mv(it, mS, uB, it.act_w);
+    if constexpr (SLDS) {
+      s_mv2<SLDS, SL>(it, mS, slds, uA, uB, t, wA_i, wB_i);
+    } else {
+      wA_i = row_mv(it, mS, uA, it.act_w);
+      wB_i = row_mv(it, mS, uB, it.act_w);
+    }
     it.publish_edge(wA_i, 2);
     it.publish_edge(wB_i, 2, VB);
     it.reduce_publish6(it.owner ? rA_i * uA_i : (T)0, it.owner ? wA_i * uA_i : (T)0, it.owner ? rA_i * rA_i : (T)0,
@@ -1463,8 +1587,12 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
       uB[(m + 1) * nP + row] = uB_i;
     }
     __syncthreads();
-    wA_i = row_mv(it, mS, uA, it.act_w);
-    wB_i = row_mv(it, mS, uB, it.act_w);
+    if constexpr (SLDS) {
+      s_mv2<SLDS, SL>(it, mS, slds, uA, uB, t, wA_i, wB_i);
+    } else {
+      wA_i = row_mv(it, mS, uA, it.act_w);
+      wB_i = row_mv(it, mS, uB, it.act_w);
+    }
     // both edge blocks every exchange (a stopped side's are stale and unused):
     // the neighbours' gather waits for every slot of the current epoch
     it.publish_edge(wA_i, 2);
@@ -1629,10 +1757,15 @@ __global__ void __launch_bounds__(NT) pcg_rowcast_kernel(KSys one, const KSys *_
 // reg_item_pair, every other item to reg_item.
 template <typename T, int NS, int NT, bool BC>
 __device__ __forceinline__ void pair_item(const KSys &d, uint32_t rank, T *lds) {
-  if ((d.flags & (KF_FWD | KF_ADJ)) == (KF_FWD | KF_ADJ) && (d.flags & KF_FUSED))
+  if ((d.flags & (KF_FWD | KF_ADJ)) == (KF_FWD | KF_ADJ) && (d.flags & KF_FUSED)) {
     reg_item_pair<T, NS, NT, BC>(d, rank, lds);
-  else
+  } else if constexpr (pair_s_lds(sizeof(T), NT, BC, NS)) {
+    // instances with S in LDS run paired items only (plans route no other item
+    // to them): flag the launch rather than skip the item silently
+    if (threadIdx.x == 0 && d.err != nullptr) __hip_atomic_store(d.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  } else {
     reg_item<T, NS, NT, true, BC, false>(d, rank, lds);
+  }
 }
 
 template <typename T, int NS, int NT, bool BC>

@@ -57,10 +57,14 @@ bool reg_supported(int n, bool bc) {
 size_t reg_lds_bytes(int dtype, uint32_t n, uint32_t K, int nt, RegVariant v) {
   const size_t ts = dtype == GBDPCG_F64 ? 8 : 4;
   const uint32_t lpr = reg_lpr(n, nt, (int)ts);
-  const size_t VL = (size_t)(K + 4) * padded_n(n) + (v.bc ? 16u : lpr * reg_chunk(n, lpr));
+  // paired instances that keep their S chunks in LDS: pair_s_slots chunks ahead
+  // of the vectors, whose read-ahead is then 8 values
+  const bool slds = v.pair && pair_s_lds(ts, nt, v.bc, (int)n);
+  const size_t VL = (size_t)(K + 4) * padded_n(n) + (v.bc ? 16u : slds ? 8u : lpr * reg_chunk(n, lpr));
   // paired instances that keep their Pinv chunks in LDS: nt chunks ahead of the vectors
   const size_t pinv = v.pair && pair_pinv_lds(ts, nt, v.bc, (int)n) ? (size_t)nt * reg_chunk(n, lpr) : 0;
-  if (v.pair) return (pinv + 8 * VL + 6 * (size_t)(nt / 64) + 8 + 4) * ts;
+  const size_t smat = slds ? (size_t)pair_s_slots(nt, (int)n) * reg_chunk(n, lpr) : 0;
+  if (v.pair) return (pinv + smat + 8 * VL + 6 * (size_t)(nt / 64) + 8 + 4) * ts;
   return (5 * VL + 3 * (size_t)(nt / 64) + 4 + 4) * ts;
 }
 
