@@ -227,6 +227,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
     c.knots_per_wg = hit.a;
     c.threads = hit.b;
     c.kernel = hit.c;
+    c.batch_split = hit.d;
     int rc = gbdpcg_batch_create(out, dtype, systems, count, &c, 0);
     if (rc == GBDPCG_OK) {
       gbdpcg_batch_info info{};
@@ -242,8 +243,9 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
     // the cached choice no longer plans (other device limits): tune again
   }
 
-  // candidate knobs: kernel x threads per workgroup x knots-per-workgroup caps
-  // (identical plans are timed once, see the deduplication below)
+  // candidate knobs: kernel x threads per workgroup x one launch or one per
+  // kernel class x knots-per-workgroup caps (identical plans are timed once, see
+  // the deduplication below)
   uint32_t maxN = 0;
   for (uint32_t i = 0; i < count; ++i) maxN = std::max(maxN, systems[i].N);
   const uint32_t caps[] = {0, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
@@ -261,26 +263,35 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   std::vector<uint32_t> kernels = {cfg->kernel};
   if (cfg->kernel == GBDPCG_KERNEL_AUTO) kernels.push_back(GBDPCG_KERNEL_ROWCAST);
 
+  // the requested launch split first (its cap-0 plan at the default threads is the default)
+  const std::vector<uint32_t> splits = {cfg->batch_split, cfg->batch_split ? 0u : 1u};
+
   gbdpcg_batch *best = nullptr;
   double best_us = 1e300, default_us = -1.0;
-  uint32_t best_cap = 0, best_nt = 512, best_kernel = cfg->kernel;
+  uint32_t best_cap = 0, best_nt = 512, best_kernel = cfg->kernel, best_split = cfg->batch_split;
   std::vector<std::vector<uint32_t>> seen;  // per-system workgroup vectors already timed
-  std::vector<int> seen_nt, seen_k;         // ... and the threads per workgroup and kernel they ran with
+  std::vector<int> seen_nt, seen_k, seen_l;  // ... and the threads per workgroup, kernel and launches they ran with
   for (uint32_t kid : kernels)
-  for (uint32_t nt : nts) {
+  for (uint32_t nt : nts)
+  for (uint32_t sp : splits) {
     for (uint32_t cap : caps) {
       if (cap > maxN) continue;
       gbdpcg_config c = *cfg;
       c.knots_per_wg = cap;
       c.threads = nt;
       c.kernel = kid;
+      c.batch_split = sp;
       gbdpcg_batch *p = nullptr;
       if (gbdpcg_batch_create(&p, dtype, systems, count, &c, 0) != GBDPCG_OK) continue;  // does not plan
       std::vector<uint32_t> w(count);
       gbdpcg_batch_wgs(p, w.data());
+      gbdpcg_batch_info pinfo{};
+      gbdpcg_batch_info_get(p, &pinfo);
       bool dup = false;
       for (size_t j = 0; j < seen.size(); ++j)
-        if (seen[j] == w && seen_nt[j] == batch_threads(p) && seen_k[j] == batch_kernel(p)) dup = true;
+        if (seen[j] == w && seen_nt[j] == batch_threads(p) && seen_k[j] == batch_kernel(p) &&
+            seen_l[j] == (int)pinfo.launches)
+          dup = true;
       if (dup) {
         gbdpcg_batch_destroy(p);
         continue;
@@ -288,6 +299,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
       seen.push_back(w);
       seen_nt.push_back(batch_threads(p));
       seen_k.push_back(batch_kernel(p));
+      seen_l.push_back((int)pinfo.launches);
       double us = 0.0;
       rc = time_plan(p, count, snap, trials, st, &us);
       if (rc < 0) {
@@ -298,7 +310,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
         return rc;
       }
       ++rep.candidates;
-      if (cap == 0 && nt == def_nt && kid == cfg->kernel) default_us = us;
+      if (cap == 0 && nt == def_nt && kid == cfg->kernel && sp == cfg->batch_split) default_us = us;
       if (us < best_us) {
         if (best) gbdpcg_batch_destroy(best);
         best = p;
@@ -306,6 +318,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
         best_cap = cap;
         best_nt = nt;
         best_kernel = kid;
+        best_split = sp;
       } else {
         gbdpcg_batch_destroy(p);
       }
@@ -329,7 +342,7 @@ extern "C" int gbdpcg_batch_autotune(gbdpcg_batch **out, int dtype, const gbdpcg
   rep.kernel = (uint32_t)batch_kernel(best);
   rep.best_us = best_us;
   rep.default_us = default_us;
-  tune_store(key, TuneChoice{best_cap, best_nt, best_kernel, 0});
+  tune_store(key, TuneChoice{best_cap, best_nt, best_kernel, best_split});
   if (report) *report = rep;
   *out = best;
   return GBDPCG_OK;

@@ -356,7 +356,8 @@ int gbdpcg_tune_single(int dtype, uint32_t stateSize, uint32_t knotPoints, const
                        gbdpcg_tune_report *report);
 
 /* Batch: builds candidate plans of the same systems (cfg->knots_per_wg,
- * cfg->threads and the register layout varied), runs each on the systems' own data and returns the
+ * cfg->threads, the register layout and cfg->batch_split varied), runs each on
+ * the systems' own data and returns the
  * fastest plan in *out.  The forward systems' lambda is restored to its value at
  * entry; adjoint outputs hold the last trial's values.  Not allowed during graph
  * capture.  trials = timed runs per candidate (0 = 3). */

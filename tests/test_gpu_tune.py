@@ -86,6 +86,8 @@ def test_batch_autotune_and_cache(gpu):
     b2 = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, autotune=True)
     assert b2.tune_report["from_cache"] == 1 and b2.tune_report["candidates"] == 0
     assert b2.wgs() == b.wgs()
+    # the cached choice includes the launch split (one launch, or one per kernel class)
+    assert b2.info()["launches"] == b.info()["launches"]
 
 
 @pytest.mark.parametrize("cap", [2, 4, 9])
