@@ -137,6 +137,33 @@ def test_register_layouts_forward_adjoint(gpu, dtype, n, kernel):
     check(systems, host, b.results(), dtype, 1, adjoint=True)
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_fp64_n28_forward_adjoint(gpu, mixed):
+    """FP64 rows of n = 28, forward + adjoint: a plan of paired systems only runs
+    the 512-thread instance that keeps S in LDS (from one rank for N = 5 to
+    ranks of 7 knots); with one
+    forward-only system in the plan it runs the 256-thread one-lane-per-row
+    instance instead.  Every system against the oracle either way."""
+    import gbdpcg
+    specs = [(28, N, 1e-10) for N in (5, 14, 40, 97)]
+    systems, host = make_batch(specs, np.float64, adjoint=True, seed0=9700)
+    if mixed:
+        systems[1]["adjoint"] = False
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1)
+    # the S chunks take 141 KB of LDS per workgroup; the other instance ~26 KB
+    lds = b.info()["lds_bytes"]
+    assert (lds < 64 * 1024) if mixed else (lds > 140 * 1024), lds
+    b.run()
+    res = b.results()
+    if mixed:
+        check(systems[1:2], host[1:2], res[1:2], np.float64, 1)
+        keep = [0, 2, 3]
+        check([systems[i] for i in keep], [host[i] for i in keep], [res[i] for i in keep], np.float64, 1,
+              adjoint=True)
+    else:
+        check(systems, host, res, np.float64, 1, adjoint=True)
+
+
 def test_repeated_runs_and_small_lds_budget(gpu):
     """Runs continue the exchange epochs; a small LDS budget forces more workgroups
     per system and several rounds per workgroup."""
