@@ -61,7 +61,9 @@ __host__ __device__ constexpr uint32_t row_stride(uint32_t n) { return col_chunk
 // Two lanes per row halve each lane's matrix registers; one lane per row packs
 // twice the knots into a workgroup, which is what n = 28 needs (its batches are
 // bound by knots resident per CU): FP32 at 512 threads (~250 VGPRs), FP64 at 256
-// threads (one wave per SIMD, matrix chunks partly in AGPRs).
+// threads (one wave per SIMD, matrix chunks partly in AGPRs).  FP64 rows of
+// n = 28 at 512 threads take two lanes per row: the paired instance there keeps
+// S in LDS (the same 7 knots per workgroup, two waves per SIMD).
 __host__ __device__ constexpr uint32_t reg_lpr(uint32_t n, int nt, int tsize) {
   return n <= 2 ? 1u : (n == 28 && ((nt == 512 && tsize == 4) || (nt == 256 && tsize == 8)) ? 1u : 2u);
 }
