@@ -835,6 +835,42 @@ __device__ __forceinline__ void s_mv2(const RegItem<T, NS, NT, BC> &it, const T 
   }
 }
 
+// g_S block rows of a rank's owned knots a..a+K-1: -1/2 (gg_k lam_j^T +
+// lam_k gg_j^T) for j in {k-1, k, k+1}, zero outside the system.  gg and lam are
+// the ghost-extended windows in LDS (knot a-1+i at (i+1) nP).  V consecutive rows
+// of one block column are contiguous in the cbtd layout, so each thread forms V
+// of them and stores them at once (16 bytes where n and gS's alignment allow).
+template <typename T, int NS, int NT>
+__device__ __forceinline__ void write_gS(T *gS, const T *gg, const T *lam, int a, int K, int N) {
+  constexpr int n = NS, nP = (int)padded_n(NS), bs = 3 * NS * NS;
+  constexpr int VM = (int)(16 / sizeof(T));
+  constexpr int V = n % VM == 0 ? VM : (n % 2 == 0 ? 2 : 1);
+  typedef T vec_t __attribute__((ext_vector_type(V)));
+  const bool vec_ok = V > 1 && ((uintptr_t)gS & (V * sizeof(T) - 1)) == 0;
+  const int step = vec_ok ? V : 1;
+  for (int e = opaque_tid() * step; e < K * bs; e += NT * step) {
+    const int kl = e / bs;
+    const int wq = e - kl * bs;
+    const int col = wq / n;
+    const int rr_ = wq - col * n;
+    const int seg = col / n;
+    const int cc = col - seg * n;
+    const int k = a + kl;
+    const int j = k - 1 + seg;
+    const bool in = j >= 0 && j < N;
+    const int lk = (kl + 2) * nP, lj = (kl + 1 + seg) * nP;
+    const T lj_v = in ? lam[lj + cc] : (T)0, gj_v = in ? gg[lj + cc] : (T)0;
+    if (vec_ok) {
+      vec_t out;
+#pragma unroll
+      for (int v = 0; v < V; ++v) out[v] = in ? (T)-0.5 * (gg[lk + rr_ + v] * lj_v + lam[lk + rr_ + v] * gj_v) : (T)0;
+      *reinterpret_cast<__attribute__((address_space(1))) vec_t *>(gptr(gS) + (size_t)a * bs + e) = out;
+    } else {
+      gptr(gS)[(size_t)a * bs + e] = in ? (T)-0.5 * (gg[lk + rr_] * lj_v + lam[lk + rr_] * gj_v) : (T)0;
+    }
+  }
+}
+
 // PRE: the set-up's HBM reads (initial guess window, first right-hand side)
 // issued together with the matrix chunks, one round trip instead of three (off
 // inside the paired kernel, whose 128-VGPR allocation spills differently, and
@@ -1234,25 +1270,7 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       }
     }
     if (pass == 1) {
-      // g_S block rows of the own knots: -1/2 (gg_k lam_j^T + lam_k gg_j^T), j in {k-1,k,k+1}
-      const int bs = 3 * n * n;
-      T *gS = static_cast<T *>(d.gS);
-      for (int e = opaque_tid(); e < K * bs; e += NT) {
-        const int kl = e / bs;
-        const int wq = e - kl * bs;
-        const int col = wq / n;
-        const int rr_ = wq - col * n;
-        const int seg = col / n;
-        const int cc = col - seg * n;
-        const int k = it.a + kl;
-        const int j = k - 1 + seg;
-        T val = 0;
-        if (j >= 0 && j < it.N) {
-          const int lk = (kl + 2) * nP, lj = (kl + 1 + seg) * nP;
-          val = (T)-0.5 * (it.xv[lk + rr_] * it.lv[lj + cc] + it.lv[lk + rr_] * it.xv[lj + cc]);
-        }
-        gptr(gS)[(size_t)it.a * bs + e] = val;
-      }
+      write_gS<T, NS, NT>(static_cast<T *>(d.gS), it.xv, it.lv, it.a, K, it.N);
     }
     __syncthreads();
   }
@@ -1635,25 +1653,7 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
       fo.status = GBDPCG_TIMED_OUT;
       ao.status = GBDPCG_TIMED_OUT;
     } else {
-      // g_S block rows of the own knots: -1/2 (gg_k lam_j^T + lam_k gg_j^T), j in {k-1,k,k+1}
-      const int bs = 3 * n * n;
-      T *gS = static_cast<T *>(d.gS);
-      for (int e = opaque_tid(); e < K * bs; e += NT) {
-        const int kl = e / bs;
-        const int wq = e - kl * bs;
-        const int col = wq / n;
-        const int rr_ = wq - col * n;
-        const int seg = col / n;
-        const int cc = col - seg * n;
-        const int k = it.a + kl;
-        const int j = k - 1 + seg;
-        T val = 0;
-        if (j >= 0 && j < it.N) {
-          const int lk = (kl + 2) * nP, lj = (kl + 1 + seg) * nP;
-          val = (T)-0.5 * (it.xv[lk + rr_] * it.lv[lj + cc] + it.lv[lk + rr_] * it.xv[lj + cc]);
-        }
-        gptr(gS)[(size_t)it.a * bs + e] = val;
-      }
+      write_gS<T, NS, NT>(static_cast<T *>(d.gS), it.xv, it.lv, it.a, K, it.N);
     }
   }
   __syncthreads();
