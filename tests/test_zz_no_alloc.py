@@ -75,7 +75,11 @@ def test_noalloc_matches_async_solve(gpu, dtype, recurrence):
 def test_noalloc_rejects_what_does_not_fit(gpu):
     import torch
     import gbdpcg
-    n, N = 14, 4096  # 512 workgroups of records: more than the 1 MiB scratch holds
+    # 512 workgroups of records, more than the 1 MiB scratch holds: the layout of
+    # the 512-thread FP32 n = 14 register instance at two workgroups per CU (it
+    # must stay within 128 VGPRs; with fewer co-resident workgroups the solve
+    # falls to the LDS kernel, whose records fit)
+    n, N = 14, 4096
     S, P, g = _system(n, N, 21, np.float32)
     lam = torch.full((n * N,), 2.0, device="cuda")
     rec = torch.zeros(gbdpcg.RESULT_BYTES, dtype=torch.uint8, device="cuda")
