@@ -551,7 +551,7 @@ __device__ __forceinline__ T group_sum(T v) {
 
 // Loads this lane's chunk of row `row` of block row kg of a cbtd matrix
 // (padded row-major columns [c*CW, (c+1)*CW) of the n x 3nP block row).
-constexpr int kLoadTables = 0, kLoadArith = 1, kLoadGuarded = 2;
+constexpr int kLoadTables = 0, kLoadArith = 1, kLoadGuarded = 2, kLoadRows = 3;
 
 // Per-chunk column tables of a block row (lane chunk c of CW padded columns):
 // the element offset of each column (0 for a padded column) and bit masks of
@@ -586,8 +586,10 @@ __host__ __device__ constexpr ChunkTables<NS, CW> chunk_tables() {
 // branch-free, offsets computed per element (other fixed sizes n <= 14);
 // kLoadGuarded: one guarded load per element (an exec-mask branch each, so the
 // loads issue slowly), kept where the branch-free forms' live masks cost more
-// than they save (the paired kernel's 128-VGPR allocation, the runtime-n and
-// n = 28 batch instances).
+// than they save (the paired kernel's 128-VGPR allocation, the runtime-n
+// instances); kLoadRows: whole rows (one lane per row, CW = 3 nP), whose guards
+// are uniform over each block of n columns: one branch per block (the n = 28
+// one-lane-per-row instances).
 template <typename T, int NS, int CW, int MODE = kLoadTables>
 __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A, int kg, int row, int c, int N,
                                            bool active) {
@@ -596,7 +598,21 @@ __device__ __forceinline__ void load_chunk(T (&dst)[CW], const T *__restrict__ A
   // batched item loop (they would stay live, in registers, for the whole solve)
   int cl = c, rl = row;
   asm volatile("" : "+v"(cl), "+v"(rl));
-  if constexpr (MODE == kLoadGuarded) {
+  if constexpr (MODE == kLoadRows) {
+    static_assert(CW == 3 * nP, "whole-row loads");
+    const bool rowok = active && kg >= 0 && kg < N;
+#pragma unroll
+    for (int s3 = 0; s3 < 3; ++s3) {
+      if (rowok && !(kg == 0 && s3 == 0) && !(kg == N - 1 && s3 == 2)) {
+#pragma unroll
+        for (int cc = 0; cc < nP; ++cc)
+          dst[s3 * nP + cc] = cc < n ? gptr(A)[(size_t)kg * 3 * n * n + (size_t)(s3 * n + cc) * n + rl] : (T)0;
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < nP; ++cc) dst[s3 * nP + cc] = (T)0;
+      }
+    }
+  } else if constexpr (MODE == kLoadGuarded) {
 #pragma unroll
     for (int j = 0; j < CW; ++j) {
       const int col = cl * CW + j;
@@ -835,7 +851,7 @@ __device__ __forceinline__ void s_mv2(const RegItem<T, NS, NT, BC> &it, const T 
   }
 }
 
-// g_S block rows of a rank's owned knots a..a+K-1: -1/2 (gg_k lam_j^T +
+// g_S block rows of a rank's owned knots a..a+K-1 (paired items): -1/2 (gg_k lam_j^T +
 // lam_k gg_j^T) for j in {k-1, k, k+1}, zero outside the system.  gg and lam are
 // the ghost-extended windows in LDS (knot a-1+i at (i+1) nP).  V consecutive rows
 // of one block column are contiguous in the cbtd layout, so each thread forms V
@@ -974,7 +990,8 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
   // layouts); the runtime-n and n = 28 instances keep the guarded form, whose
   // fewer live masks leave their batches faster
   constexpr bool kGuarded = !(BF && NS <= 14);
-  constexpr int kMode = kGuarded ? kLoadGuarded : (sizeof(T) == 4 && NT == 256) ? kLoadTables : kLoadArith;
+  constexpr int kMode = kGuarded ? (R::LPR == 1 && CW == 3 * nP ? kLoadRows : kLoadGuarded)
+                                 : (sizeof(T) == 4 && NT == 256) ? kLoadTables : kLoadArith;
   // S first: loads complete in issue order, so the first product r = b - S x
   // (and the first exchange) can start while the Pinv chunks are in flight
   if constexpr (BC) {
@@ -1270,7 +1287,28 @@ __device__ __forceinline__ void reg_item(const KSys &d, uint32_t rank, T *lds) {
       }
     }
     if (pass == 1) {
-      write_gS<T, NS, NT>(static_cast<T *>(d.gS), it.xv, it.lv, it.a, K, it.N);
+      // g_S block rows of the own knots: -1/2 (gg_k lam_j^T + lam_k gg_j^T), j in {k-1,k,k+1}
+      // (one element per thread and step: the vectorised write_gS costs this
+      // item two VGPRs, and with them the 512-thread FP32 n = 14 instance's
+      // second workgroup per CU)
+      const int bs = 3 * n * n;
+      T *gS = static_cast<T *>(d.gS);
+      for (int e = opaque_tid(); e < K * bs; e += NT) {
+        const int kl = e / bs;
+        const int wq = e - kl * bs;
+        const int col = wq / n;
+        const int rr_ = wq - col * n;
+        const int seg = col / n;
+        const int cc = col - seg * n;
+        const int k = it.a + kl;
+        const int j = k - 1 + seg;
+        T val = 0;
+        if (j >= 0 && j < it.N) {
+          const int lk = (kl + 2) * nP, lj = (kl + 1 + seg) * nP;
+          val = (T)-0.5 * (it.xv[lk + rr_] * it.lv[lj + cc] + it.lv[lk + rr_] * it.xv[lj + cc]);
+        }
+        gptr(gS)[(size_t)it.a * bs + e] = val;
+      }
     }
     __syncthreads();
   }
@@ -1400,8 +1438,9 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
     }
     load_chunk<T, NS, CW, kLoadGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
   } else if constexpr (!PLDS) {
-    load_chunk<T, NS, CW, kLoadGuarded>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
-    load_chunk<T, NS, CW, kLoadGuarded>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
+    constexpr int kPairLoad = R::LPR == 1 && CW == 3 * nP ? kLoadRows : kLoadGuarded;
+    load_chunk<T, NS, CW, kPairLoad>(mM, static_cast<const T *>(d.Pinv), kg, row, c, it.N, it.act_u);
+    load_chunk<T, NS, CW, kPairLoad>(mS, static_cast<const T *>(d.S), kg, row, c, it.N, it.act_w);
   } else {
     // this lane's Pinv chunk through registers into its LDS slots (read back by
     // this lane only: no barrier needed between the store and the reads)
