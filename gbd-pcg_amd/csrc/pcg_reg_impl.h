@@ -1765,13 +1765,13 @@ __device__ __forceinline__ void reg_entry(const KSys &one, const KSys *__restric
   }
 }
 
-// Waves per SIMD the column-chunk kernel is compiled for: the FP32 runtime-n
-// instance for rows of n <= 14 (NS = -1) at 512 threads is held to 128 VGPRs,
-// so that two workgroups share a CU in batch plans (the specialised ones fit
-// by themselves).
+// Waves per SIMD the column-chunk kernel is compiled for: the FP32 instances
+// for rows of n <= 14 at 512 threads (the specialised ones and the runtime-n
+// one, NS = -1) are held to 128 VGPRs, so that two workgroups share a CU in
+// batch plans and long single solves (the planners count on it).
 template <typename T, int NS, int NT>
 constexpr int reg_waves_per_eu() {
-  return (sizeof(T) == 4 && NT == 512 && NS == -1) ? 4 : 1;
+  return (sizeof(T) == 4 && NT == 512 && NS != 0 && NS <= 14) ? 4 : 1;
 }
 
 // Register-resident rows, reg_lpr(n) lanes per row, column chunks.
