@@ -134,6 +134,13 @@ bool pair28(int dtype, int nt, bool all_pair) {
   return dtype == GBDPCG_F64 ? (nt == 256 || (nt == 512 && all_pair)) : nt == 512;
 }
 
+// Whether the FP64 n = 28 paired instance with S in LDS (512 threads, up to 7
+// knots per workgroup) fits an LDS budget per workgroup.
+bool s_lds_fits(size_t budget) {
+  const size_t need = reg_lds_bytes(GBDPCG_F64, 28, 7, 512, RegVariant{true, false, true});
+  return ((need + 255) & ~(size_t)255) <= budget;
+}
+
 bool all_paired(const gbdpcg_system_desc *systems, const std::vector<uint32_t> &idx) {
   for (uint32_t i : idx)
     if (!((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT))) return false;
@@ -192,6 +199,11 @@ int plan_launch(int dtype, const gbdpcg_system_desc *systems, const std::vector<
     }
   }
   lds = (lds + 255) & ~(size_t)255;
+  if (L.reg && lds > budget) {
+    set_error("the register kernel's layout needs %zu B of LDS per workgroup, above the budget of %zu B", lds,
+              budget);
+    return GBDPCG_ERR_TOO_LARGE;
+  }
   int occ = 0;
   if (L.reg) {
     HIPCHK(reg_prepare(dtype, L.kn, L.nt, lds, rv));
@@ -386,7 +398,8 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     // FP64 whole rows (row broadcast) and FP64 rows of n = 28: 256 threads, but
     // for a launch of forward + adjoint systems of n = 28 only, 512 (S in LDS)
     const bool pair_only = all_paired(systems, small);
-    const bool f64_pair28 = dtype == GBDPCG_F64 && one_n && systems[small[0]].n == 28 && pair_only && !bc;
+    const bool f64_pair28 = dtype == GBDPCG_F64 && one_n && systems[small[0]].n == 28 && pair_only && !bc &&
+                            s_lds_fits(budget);
     const bool only256 = dtype == GBDPCG_F64 && (bc || has_wide) && !f64_pair28;
     L.nt = (one_n && (want256 || only256)) ? 256 : 512;
     // forward + adjoint systems run both solves in one loop (the paired kernel;
@@ -402,7 +415,7 @@ int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc 
     BatchLaunch L;
     L.reg = true;
     L.kn = 28;
-    const bool pair_only = all_paired(systems, wide);
+    const bool pair_only = all_paired(systems, wide) && (dtype != GBDPCG_F64 || s_lds_fits(budget));
     L.nt = (want256 || (dtype == GBDPCG_F64 && !pair_only)) ? 256 : 512;
     for (uint32_t i : wide)
       if ((systems[i].flags & GBDPCG_SYS_FORWARD) && (systems[i].flags & GBDPCG_SYS_ADJOINT)) L.pair = true;

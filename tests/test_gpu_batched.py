@@ -137,22 +137,24 @@ def test_register_layouts_forward_adjoint(gpu, dtype, n, kernel):
     check(systems, host, b.results(), dtype, 1, adjoint=True)
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_fp64_n28_forward_adjoint(gpu, mixed):
+@pytest.mark.parametrize("variant", ["paired", "mixed", "budget"])
+def test_fp64_n28_forward_adjoint(gpu, variant):
     """FP64 rows of n = 28, forward + adjoint: a plan of paired systems only runs
     the 512-thread instance that keeps S in LDS (from one rank for N = 5 to
-    ranks of 7 knots); with one
-    forward-only system in the plan it runs the 256-thread one-lane-per-row
-    instance instead.  Every system against the oracle either way."""
+    ranks of 7 knots); with one forward-only system in the plan, or with an LDS
+    budget below what that instance needs, it runs the 256-thread
+    one-lane-per-row instance instead.  Every system against the oracle."""
     import gbdpcg
+    mixed = variant == "mixed"
     specs = [(28, N, 1e-10) for N in (5, 14, 40, 97)]
     systems, host = make_batch(specs, np.float64, adjoint=True, seed0=9700)
     if mixed:
         systems[1]["adjoint"] = False
-    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1)
+    budget = 64 * 1024 if variant == "budget" else 0
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1, lds_budget=budget)
     # the S chunks take 141 KB of LDS per workgroup; the other instance ~26 KB
     lds = b.info()["lds_bytes"]
-    assert (lds < 64 * 1024) if mixed else (lds > 140 * 1024), lds
+    assert (lds > 140 * 1024) if variant == "paired" else (lds < 64 * 1024), lds
     b.run()
     res = b.results()
     if mixed:
