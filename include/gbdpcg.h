@@ -305,7 +305,10 @@ typedef struct gbdpcg_batch_info {
 } gbdpcg_batch_info;
 
 /* cfg supplies exit_mode, recurrence and stream; tolerance/max_iter are per system.
- * lds_budget_bytes: 0 = one workgroup per CU (the whole LDS); else the cap per WG. */
+ * lds_budget_bytes: 0 = the whole LDS of a CU; else the cap per workgroup (the
+ * LDS kernel takes more workgroups per system to stay under it; a register
+ * layout that needs more is replaced where an alternative exists, else the plan
+ * is rejected with GBDPCG_ERR_TOO_LARGE). */
 int gbdpcg_batch_create(gbdpcg_batch **out, int dtype, const gbdpcg_system_desc *systems,
                         uint32_t count, const gbdpcg_config *cfg, uint32_t lds_budget_bytes);
 int gbdpcg_batch_run(gbdpcg_batch *plan, void *stream);
