@@ -36,22 +36,35 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def check(systems, host, results, dtype, recurrence, adjoint=False):
+def check(systems, host, results, dtype, recurrence, adjoint=False, mode=O.EXIT_REL):
+    """Every system against the oracle: the iteration count within one of the
+    oracle's at the same precision (both forms of the recurrence for the fused
+    one), lambda (and g_gamma, g_S for adjoint systems) at the oracle's iterate
+    of the same count.  mode: the exit rule the plan ran with."""
     tol = 1e-4 if dtype == np.float32 else 1e-10
+
+    def counts(rhs, tol_sys):
+        z = np.zeros(rhs.size, dtype)
+        _, it_same, *_ = O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), z, n, N, tol_sys, 173, mode,
+                               dtype, fused=recurrence == 1)
+        its = [it_same]
+        if recurrence == 1:  # the fused form against the reference iteration's own count as well
+            _, it_refform, *_ = O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), z, n, N, tol_sys, 173,
+                                      mode, dtype, fused=False)
+            its.append(it_refform)
+        return its
+
     for d, (S, P, g, glam), r in zip(systems, host, results):
         n, N = d["n"], d["N"]
         assert r.status == 0, (n, N, r.status)
-        _, it_same, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N,
-                               d["tol"], 173, 1, dtype, fused=recurrence == 1)
-        assert abs(r.iters - it_same) <= 1, (n, N, r.iters, it_same)
-        if recurrence == 1:  # the fused form against the reference iteration's own count as well
-            _, it_refform, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n,
-                                      N, d["tol"], 173, 1, dtype, fused=False)
-            assert abs(r.iters - it_refform) <= 1, (n, N, r.iters, it_refform)
+        for it in counts(g, d["tol"]):
+            assert abs(r.iters - it) <= 1, (n, N, r.iters, it)
         lam_ref, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 0.0, r.iters)
         assert rel(d["lam"].cpu().numpy(), lam_ref) <= tol, (n, N)
         if adjoint:
             assert r.adj_status == 0
+            for it in counts(glam, d["tol"]):
+                assert abs(r.adj_iters - it) <= 1, (n, N, "adjoint", r.adj_iters, it)
             gg_ref, *_ = O.pcg(S, P, glam, np.zeros(n * N), n, N, 0.0, r.adj_iters)
             gg = d["ggamma"].cpu().numpy().astype(np.float64)
             assert rel(gg, gg_ref) <= tol
@@ -164,6 +177,21 @@ def test_fp64_n28_forward_adjoint(gpu, variant):
               adjoint=True)
     else:
         check(systems, host, res, np.float64, 1, adjoint=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_batch_reference_exit_rule(gpu, dtype):
+    """Batch plans under the reference's exit rule |r'z| < tol
+    (include/pcg.cuh:195): forward + adjoint systems of n in {6, 14, 28}, each
+    side's count within one of the oracle's (both recurrence forms), lambda,
+    g_gamma and g_S at the oracle's iterates."""
+    import gbdpcg
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    specs = [(6, 24, tol), (14, 40, tol), (28, 33, tol), (14, 97, tol)]
+    systems, host = make_batch(specs, dtype, adjoint=True, seed0=9800)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_ABS_ETA, recurrence=1)
+    b.run()
+    check(systems, host, b.results(), dtype, 1, adjoint=True, mode=O.EXIT_ABS)
 
 
 def test_repeated_runs_and_small_lds_budget(gpu):
