@@ -49,10 +49,13 @@ def rel(a, b):
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("recurrence", [0, 1])
 @pytest.mark.parametrize("N", [32, 128])
-def test_adjoint_against_oracle(gpu, dtype, recurrence, N):
+@pytest.mark.parametrize("mode", [1, 0])
+def test_adjoint_against_oracle(gpu, dtype, recurrence, N, mode):
+    """mode 1: relative residual; mode 0: the reference's |r'z| < tol
+    (include/pcg.cuh:195)."""
     n = 14
     S, P, g, glam = system(n, N, 4000 + N, dtype)
-    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=1e-5, max_iter=173, exit_mode=1,
+    lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=1e-5, max_iter=173, exit_mode=mode,
                                    recurrence=recurrence)
     assert res.status == 0 and res.adj_status == 0
     tol = 1e-4 if dtype == np.float32 else 1e-10
@@ -62,15 +65,15 @@ def test_adjoint_against_oracle(gpu, dtype, recurrence, N):
     assert rel(gg, gg_ref) <= tol
     # the iteration counts of both solves are within 1 of the same-precision oracle
     _, it_f, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5, 173,
-                        1, dtype, fused=recurrence == 1)
+                        mode, dtype, fused=recurrence == 1)
     _, it_a, *_ = O.pcg(S.astype(dtype), P.astype(dtype), glam.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5,
-                        173, 1, dtype, fused=recurrence == 1)
+                        173, mode, dtype, fused=recurrence == 1)
     assert abs(res.iters - it_f) <= 1 and abs(res.adj_iters - it_a) <= 1
     if recurrence == 1:  # and against the reference iteration's own counts (include/pcg.cuh:118-208)
         _, it_fr, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), np.zeros(n * N, dtype), n, N, 1e-5,
-                             173, 1, dtype, fused=False)
+                             173, mode, dtype, fused=False)
         _, it_ar, *_ = O.pcg(S.astype(dtype), P.astype(dtype), glam.astype(dtype), np.zeros(n * N, dtype), n, N,
-                             1e-5, 173, 1, dtype, fused=False)
+                             1e-5, 173, mode, dtype, fused=False)
         assert abs(res.iters - it_fr) <= 1 and abs(res.adj_iters - it_ar) <= 1, (res.iters, it_fr, res.adj_iters,
                                                                                   it_ar)
     # g_S is the masked, symmetrised outer product of the returned vectors
@@ -80,6 +83,8 @@ def test_adjoint_against_oracle(gpu, dtype, recurrence, N):
     assert np.allclose(gS, expect * mask, rtol=1e-5 if dtype == np.float32 else 1e-12,
                        atol=1e-6 * np.abs(expect).max())
     assert not gS.reshape(N, 3, n * n)[0, 0].any() and not gS.reshape(N, 3, n * n)[-1, 2].any()
+    if mode != 1:
+        return
     # against the exact FP64 gradient (block-LU solves): both solves stop at a 1e-5
     # relative residual, so the gradient is good to that order
     x = O.block_solve(S, g, n, N)
