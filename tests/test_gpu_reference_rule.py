@@ -2,7 +2,7 @@
 example shape, the block-Jacobi preconditioner, and the device-scope exchange.
 
 * Exit test |r'z| < tol (include/pcg.cuh:195) at n=14, N in {32, 64, 128, 256},
-  both recurrences, FP32 and FP64: iteration count within +-1 of the oracle at the
+  both recurrences, FP32 and FP64, register- and LDS-resident kernels: iteration count within +-1 of the oracle at the
   same precision (and, for the fused form, of the reference form), lambda within
   1e-4 / 1e-10 of the FP64 oracle run for the same number of iterations.
 * The reference defaults, tol 1e-6 and max_iter 25 (include/constants.cuh:14-16):
@@ -27,11 +27,15 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("N", [32, 64, 128, 256])
 @pytest.mark.parametrize("recurrence", [0, 1])
-def test_abs_eta_rule_n14(gpu, dtype, N, recurrence):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_abs_eta_rule_n14(gpu, dtype, N, recurrence, kernel):
+    """kernel 0: the automatic choice (register-resident rows); 1: the
+    LDS-resident kernel."""
     import gbdpcg
     n = 14
     S, P, g = lqr_system(n, N, 2000 + N, dtype)
-    res, lam, it_same = check_against_oracle(S, P, g, n, N, dtype, 1e-5, 173, gbdpcg.EXIT_ABS_ETA, recurrence)
+    res, lam, it_same = check_against_oracle(S, P, g, n, N, dtype, 1e-5, 173, gbdpcg.EXIT_ABS_ETA, recurrence,
+                                             kernel=kernel)
     assert res.status == gbdpcg.CONVERGED
     # the reference rule: the final r'z is below the tolerance in absolute value
     assert abs(res.eta) < 1e-5
