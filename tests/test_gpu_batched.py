@@ -180,6 +180,26 @@ def test_fp64_n28_forward_adjoint(gpu, variant):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gS_unaligned_output(gpu, dtype):
+    """g_S is stored 16 bytes at a time where its address allows: a g_S buffer
+    offset by one element from a 16-byte boundary takes the element-wise
+    stores, with the same values (checked like every other system)."""
+    import torch
+    import gbdpcg
+    tol = 1e-5 if dtype == np.float32 else 1e-10
+    specs = [(28, 40, tol), (28, 33, tol), (14, 32, tol)]
+    systems, host = make_batch(specs, dtype, adjoint=True, seed0=9900)
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    for d in systems[:2]:
+        n, N = d["n"], d["N"]
+        d["gS"] = torch.zeros(3 * n * n * N + 1, dtype=tdt, device="cuda")[1:]
+        assert d["gS"].data_ptr() % 16 != 0
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=1)
+    b.run()
+    check(systems, host, b.results(), dtype, 1, adjoint=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_batch_reference_exit_rule(gpu, dtype):
     """Batch plans under the reference's exit rule |r'z| < tol
     (include/pcg.cuh:195): forward + adjoint systems of n in {6, 14, 28}, each
