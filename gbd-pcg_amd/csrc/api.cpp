@@ -436,6 +436,16 @@ static int choose_layout(int dev, int dtype, uint32_t n, uint32_t N, bool fused,
   return rc;
 }
 
+// Whether the LDS-resident kernel cannot give every workgroup the 2 knots the
+// fused recurrence needs: at most N / 2 workgroups, so odd N leaves one of them
+// 3 knots, whose S and Phi^-1 rows exceed the LDS of a CU in FP64 from n = 25 on.
+static bool lds_fused_unfit(int dtype, uint32_t n, uint32_t N) {
+  DeviceInfo di;
+  if (N < 2 || device_info(cur_dev(), &di) < 0) return false;
+  const uint32_t G = N / 2;
+  return item_lds_bytes(n, ceil_div(N, G), G, tsize(dtype)) > di.max_lds;
+}
+
 // choose_layout for a workspace's calls, remembered per workspace: a repeated
 // solve of the same shape and config skips the tuner-cache key, the device
 // queries and the occupancy check (microseconds on every call otherwise).  The
@@ -556,9 +566,15 @@ static int solve_locked(Workspace *ws, int dtype, uint32_t n, uint32_t N, const 
   bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED;
   if (fused && N < 2) fused = false;  // one knot: the reference recurrence
   // forward + adjoint on a register kernel (n <= 14): both solves in one loop
-  const bool pair = fused && (flags & KF_FWD) && (flags & KF_ADJ) && n <= 14;
+  bool pair = fused && (flags & KF_FWD) && (flags & KF_ADJ) && n <= 14;
   Layout L;
   int rc = cached_layout(ws, dtype, n, N, fused, cfg, &L, pair);
+  if (rc == GBDPCG_ERR_TOO_LARGE && fused && lds_fused_unfit(dtype, n, N)) {
+    // no layout with 2 knots per workgroup: the reference recurrence, which
+    // takes one knot per workgroup (batch plans decide the same per system)
+    fused = pair = false;
+    rc = cached_layout(ws, dtype, n, N, false, cfg, &L, false);
+  }
   if (rc < 0) return rc;
 
   if (Pinv == nullptr) {
@@ -703,6 +719,10 @@ int solve_on_scratch(int dtype, uint32_t n, uint32_t N, const void *S, const voi
   if (fused && N < 2) fused = false;
   Layout L;
   rc = choose_layout(dev, dtype, n, N, fused, cfg, &L, false);
+  if (rc == GBDPCG_ERR_TOO_LARGE && fused && lds_fused_unfit(dtype, n, N)) {
+    fused = false;  // as in solve_locked
+    rc = choose_layout(dev, dtype, n, N, false, cfg, &L, false);
+  }
   if (rc < 0) return rc;
   const size_t words = 2 * (size_t)L.G * rec_words(n, (uint32_t)(tsize(dtype) / 4));
   if (words > scratch_words - kScratchHeaderWords) {

@@ -19,7 +19,7 @@ def make_batch(specs, dtype, adjoint=False, seed0=9000):
     import gbdpcg
     systems, host = [], []
     for i, (n, N, tol) in enumerate(specs):
-        S, g = gbdpcg.lqr_generate(n, n // 2, N, seed0 + i, dtype=np.float64)
+        S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, seed0 + i, dtype=np.float64)
         P = O.precond(S, n, N, 2)
         glam = gbdpcg.randn(n * N, seed0 + 500 + i, dtype=np.float64)
         d = dict(S=_t(S.astype(dtype), dtype), Pinv=_t(P.astype(dtype), dtype), gamma=_t(g.astype(dtype), dtype),
@@ -86,6 +86,30 @@ def test_uneven_batch(gpu, dtype, recurrence):
     assert info["systems"] == 24 and info["grid"] > 0
     b.run()
     check(systems, host, b.results(), dtype, recurrence)
+
+
+def every_size_specs(dtype):
+    """One system of every stateSize 1..32, N drawn from 5..40.  FP32 to a relative
+    residual of 3e-5, above the FP32 floor of the smallest systems."""
+    rng = np.random.default_rng(32)
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    return [(n, int(rng.integers(5, 41)), tol) for n in range(1, 33)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("split", [False, True])
+def test_batch_every_state_size(gpu, dtype, split):
+    """Forward + adjoint systems of every stateSize 1..32 in one plan (fused
+    recurrence): one launch, or one per kernel class (register rows of n <= 14,
+    rows of n = 28, the LDS kernel for the sizes the register kernels do not take).
+    Every system against the oracle."""
+    import gbdpcg
+    specs = every_size_specs(dtype)
+    systems, host = make_batch(specs, dtype, adjoint=True, seed0=9800)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, split_launches=split)
+    assert b.info()["systems"] == len(specs)
+    b.run()
+    check(systems, host, b.results(), dtype, 1, adjoint=True)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])

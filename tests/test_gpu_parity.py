@@ -129,6 +129,26 @@ def test_state_sizes(gpu, n):
     check_against_oracle(S, P, g, n, N, dtype, 1e-9, 200, 1, 0)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_every_state_size(gpu, dtype, recurrence, kernel):
+    """Every stateSize 1..32 at N = 5 and 24, on the automatic choice (register
+    instances for n in {2, 6, 12, 14, 28}, the LDS kernel otherwise) and forced onto
+    the LDS kernel.  FP32 runs to a relative residual of 3e-5: at 1e-5 the smallest
+    systems sit at the FP32 floor (n = 1, N = 5: the oracle's fused and reference
+    forms stop at 3 and 6 iterations), where the two recurrences' residuals straddle
+    the threshold.  FP64 fused solves of n >= 25 at N = 5 have no LDS layout with
+    2 knots per workgroup and run the reference iteration (the count is held to
+    both forms' either way)."""
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    for n in range(1, 33):
+        for N in (5, 24):
+            S, P, g = lqr_system(n, N, 1000 + 37 * n + N, dtype)
+            res, _, _ = check_against_oracle(S, P, g, n, N, dtype, tol, 400, 1, recurrence, kernel=kernel)
+            assert res.status == 0, (n, N, res.status)
+
+
 def test_edge_cases(gpu):
     import gbdpcg
     n, N, dtype = 4, 5, np.float64
