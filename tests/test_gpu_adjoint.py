@@ -23,7 +23,7 @@ def _t(a, dtype):
 
 def system(n, N, seed, dtype):
     import gbdpcg
-    S, g = gbdpcg.lqr_generate(n, n // 2, N, seed, dtype=np.float64)
+    S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, seed, dtype=np.float64)
     P = O.precond(S, n, N, 2)
     glam = gbdpcg.randn(n * N, seed + 99, dtype=np.float64)
     return S, P, g, glam
@@ -93,6 +93,51 @@ def test_adjoint_against_oracle(gpu, dtype, recurrence, N, mode):
     print(f"N={N} {dtype.__name__} rec={recurrence}: iters {res.iters}/{res.adj_iters} "
           f"g_gamma err vs exact {rel(gg, gg_x):.2e}, g_S err {rel(gS, gS_x):.2e}")
     assert rel(gg, gg_x) <= 1e-4 and rel(gS, gS_x) <= 1e-4
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_adjoint_every_state_size(gpu, dtype, recurrence):
+    """Forward + adjoint solves of every stateSize 1..32 at N in {2, 3, 5, 24}
+    (fused and n <= 14 with N >= 4: both solves in one loop on the register
+    kernel; else the LDS kernel, one after the other) against the oracle: both
+    counts within one of the same-precision oracle's (both forms for the fused
+    recurrence), lambda and g_gamma at the FP64 iterate of the same count, g_S by
+    the formula.  FP32 to a relative residual of 3e-5 (above the FP32 floor of
+    the smallest systems).  FP64 fused requests of n >= 25 at N in {3, 5}, and of
+    n = 32 beyond N = 2, have no LDS layout with 2 knots per workgroup and run the
+    reference iteration."""
+    import gbdpcg
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    # FP64: as for single solves of every size, the summation order alone moves
+    # the iterate of the least well-conditioned small systems by ~1e-10
+    lam_tol = 1e-4 if dtype == np.float32 else 1e-9
+    for n in range(1, 33):
+        for N in (2, 3, 5, 24):
+            S, P, g, glam = system(n, N, 6000 + 37 * n + N, dtype)
+            lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=tol, max_iter=400, exit_mode=1,
+                                           recurrence=recurrence)
+            assert res.status == 0 and res.adj_status == 0, (n, N, res.status, res.adj_status)
+            if recurrence == 0:
+                ran = False
+            elif n in (2, 6, 12, 14, 28) and N >= 4:
+                ran = True
+            else:
+                G = N // 2
+                ran = gbdpcg.lds_bytes(dtype, n, -(-N // G), G) <= 160 * 1024
+            for rhs, its, x in ((g, res.iters, lam), (glam, res.adj_iters, gg)):
+                z = np.zeros(n * N, dtype)
+                forms = (True, False) if recurrence == 1 else (False,)
+                for f in forms:
+                    _, it, *_ = O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), z, n, N, tol, 400, 1,
+                                      dtype, fused=f)
+                    assert abs(its - it) <= 1, (n, N, its, it, f)
+                ref, it_ref, *_ = O.pcg(S, P, rhs, np.zeros(n * N), n, N, 0.0, its, 1, np.float64, fused=ran)
+                assert it_ref == its
+                assert rel(x, ref) <= lam_tol, (n, N, rel(x, ref))
+            expect = O.dense_to_cbtd(-0.5 * (np.outer(gg, lam) + np.outer(lam, gg)), n, N) * O.pattern_mask(n, N)
+            assert np.allclose(gS, expect, rtol=1e-5 if dtype == np.float32 else 1e-12,
+                               atol=1e-6 * max(np.abs(expect).max(), 1e-300)), (n, N)
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
