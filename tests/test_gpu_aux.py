@@ -54,6 +54,47 @@ def test_precond_device_matches_definition(gpu, dtype, kind, n):
     assert rel(P, ref) <= (1e-4 if dtype == np.float32 else 1e-12)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kind", [1, 2])
+def test_precond_device_every_size(gpu, dtype, kind):
+    """Every n from 1 to 40 and n = 48 (the one-wave register inversions of
+    n <= 32 at each of their four widths, and the LDS Gauss-Jordan form above
+    32), FP32 also n = 64, against the FP64 definition; pads stay zero."""
+    import torch
+    import gbdpcg
+    N = 7
+    sizes = list(range(1, 41)) + [48] + ([64] if dtype == np.float32 else [])
+    for n in sizes:
+        S, _ = lqr(n, N, 700 + n)
+        ref = O.precond(S, n, N, kind)
+        dS = dev(S, dtype)
+        dP = torch.full_like(dS, float("nan"))
+        gbdpcg.precond_device(dS, dP, n, N, kind)
+        P = dP.cpu().numpy()
+        mask = O.pattern_mask(n, N).astype(bool)
+        assert np.all(P[~mask] == 0), n
+        assert rel(P, ref) <= (1e-5 if dtype == np.float32 else 1e-12), (n, rel(P, ref))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_lqr_schur_device_every_size(gpu, dtype):
+    """S from (A, B, Q, R) on the device for every n from 1 to 32, with m = n / 2
+    inputs (at least one) and with none, against the host construction."""
+    import torch
+    import gbdpcg
+    N = 6
+    for n in range(1, 33):
+        for m in sorted({max(n // 2, 1), 0}):
+            S_h, _, (A, B, Q, R) = gbdpcg.lqr_generate(n, m, N, 900 + n, dtype=np.float64, with_lqr=True)
+            dS = torch.full((3 * n * n * N,), float("nan"), dtype=_tdt(dtype), device="cuda")
+            gbdpcg.lqr_schur_device(dev(A, dtype), dev(B, dtype) if m else None, dev(Q, dtype),
+                                    dev(R, dtype) if m else None, dS, n, m, N)
+            got = dS.cpu().numpy()
+            assert np.isfinite(got).all(), (n, m)
+            assert rel(got, S_h) <= (1e-5 if dtype == np.float32 else 1e-13), (n, m, rel(got, S_h))
+            assert np.all(got[~O.pattern_mask(n, N).astype(bool)] == 0), (n, m)
+
+
 def test_precond_device_singular_block(gpu):
     import gbdpcg
     n, N = 3, 4
