@@ -113,6 +113,67 @@ def test_batch_every_state_size(gpu, dtype, split):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("split", [False, True])
+def test_batch_kinds_and_limits(gpu, dtype, split):
+    """One plan (fused recurrence) mixing forward-only, forward + adjoint and
+    adjoint-only systems (the last with a given lambda), per-system max_iter caps
+    far below convergence (those sides end in MAX_ITER after exactly max_iter
+    iterations, at the oracle's iterate of that count) and horizons down to N = 1
+    (one knot: the reference iteration).  FP64 lambda to 1e-9: the summation order
+    alone moves the iterate of small, less well-conditioned systems by ~1e-10."""
+    import gbdpcg
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    lam_tol = 1e-4 if dtype == np.float32 else 1e-9
+    specs = [(14, 32, "both", 173), (14, 32, "both", 5), (14, 40, "fwd", 4), (14, 24, "adj", 173),
+             (6, 3, "both", 173), (6, 2, "fwd", 173), (6, 1, "both", 173), (2, 1, "fwd", 173),
+             (28, 16, "both", 173), (28, 9, "both", 6), (7, 12, "adj", 173), (7, 3, "fwd", 2),
+             (12, 4, "both", 173), (2, 5, "adj", 3)]
+    systems, host = [], []
+    for i, (n, N, kind, mx) in enumerate(specs):
+        S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, 9900 + i, dtype=np.float64)
+        P = O.precond(S, n, N, 2)
+        glam = gbdpcg.randn(n * N, 9950 + i, dtype=np.float64)
+        lam_in = O.dense_solve(S, g, n, N) if kind == "adj" else np.zeros(n * N)
+        d = dict(S=_t(S.astype(dtype), dtype), Pinv=_t(P.astype(dtype), dtype), gamma=_t(g.astype(dtype), dtype),
+                 lam=_t(lam_in.astype(dtype), dtype), n=n, N=N, tol=tol, max_iter=mx, forward=kind != "adj",
+                 adjoint=kind != "fwd")
+        if kind != "fwd":
+            d.update(glam=_t(glam.astype(dtype), dtype), ggamma=_t(np.zeros(n * N), dtype),
+                     gS=_t(np.zeros(3 * n * n * N), dtype))
+        systems.append(d)
+        host.append((S, P, g, glam, lam_in))
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, split_launches=split)
+    assert b.info()["systems"] == len(specs)
+    b.run()
+
+    def side(S, P, rhs, n, N, mx, its, status, out):
+        z = np.zeros(n * N, dtype)
+        for f in (True, False):
+            _, it, st, *_ = O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), z, n, N, tol, mx, O.EXIT_REL,
+                                  dtype, fused=f)
+            if mx < 10:  # the cap binds far below convergence
+                assert st == gbdpcg.MAX_ITER and status == gbdpcg.MAX_ITER and its == mx, (n, N, its, st, status)
+            else:
+                assert status == gbdpcg.CONVERGED and abs(its - it) <= 1, (n, N, its, it, f)
+        ref, it_ref, *_ = O.pcg(S, P, rhs, np.zeros(n * N), n, N, 0.0, its, O.EXIT_REL, np.float64, fused=N >= 2)
+        assert it_ref == its
+        assert rel(out, ref) <= lam_tol, (n, N, rel(out, ref))
+
+    for (n, N, kind, mx), d, (S, P, g, glam, lam_in), r in zip(specs, systems, host, b.results()):
+        lam = d["lam"].cpu().numpy().astype(np.float64)
+        if kind == "adj":
+            assert np.array_equal(lam, lam_in.astype(dtype).astype(np.float64))  # lambda is an input here
+        else:
+            side(S, P, g, n, N, mx, r.iters, r.status, lam)
+        if kind != "fwd":
+            gg = d["ggamma"].cpu().numpy().astype(np.float64)
+            side(S, P, glam, n, N, mx, r.adj_iters, r.adj_status, gg)
+            expect = O.dense_to_cbtd(-0.5 * (np.outer(gg, lam) + np.outer(lam, gg)), n, N) * O.pattern_mask(n, N)
+            assert np.allclose(d["gS"].cpu().numpy(), expect, rtol=1e-5 if dtype == np.float32 else 1e-12,
+                               atol=1e-6 * np.abs(expect).max()), (n, N)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("adjoint", [False, True])
 @pytest.mark.parametrize("split", [False, True])
 def test_kernel_classes_in_one_plan(gpu, dtype, adjoint, split):
