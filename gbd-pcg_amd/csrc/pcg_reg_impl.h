@@ -889,8 +889,9 @@ __device__ __forceinline__ void write_gS(T *gS, const T *gg, const T *lam, int a
 
 // PRE: the set-up's HBM reads (initial guess window, first right-hand side)
 // issued together with the matrix chunks, one round trip instead of three (off
-// inside the paired kernel, whose 128-VGPR allocation spills differently, and
-// runs its batches slower, with it).
+// for the one-pass items of the paired kernel, whose 128-VGPR allocation spills
+// differently, and runs its batches slower, with it; the paired item issues its
+// own set-up reads together).
 // BF: the matrix chunks loaded branch-free (load_chunk), for the fixed-size
 // instances; the runtime-n instances keep the guarded loads.
 template <typename T, int NS, int NT, bool FUSED, bool BC = false, bool PRE = true, bool BF = PRE>
@@ -1419,6 +1420,29 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
   }
   for (int i = opaque_tid(); i < total; i += NT) lds[i] = (T)0;
 
+  // the set-up's vector reads (initial guess window: (K + 2) n <= NT values, one
+  // per thread; both right-hand sides of the owned rows) issued with the matrix
+  // chunks: one HBM round trip instead of three
+  const T *lam0 = static_cast<const T *>(d.lambda);
+  const int wl = (K + 2) * n;
+  T lam_pre = 0, bA = 0, bB = 0;
+  int lam_pos = -1;
+  {
+    const int i = opaque_tid();
+    if (i < wl) {
+      const int jj = i / n, e = i - jj * n;
+      const int k = it.a - 1 + jj;
+      if (k >= 0 && k < it.N) {
+        lam_pre = gptr(lam0)[(size_t)k * n + e];
+        lam_pos = (jj + 1) * nP + e;
+      }
+    }
+    if (it.owner) {
+      bA = gptr(static_cast<const T *>(d.gamma))[(size_t)kg * n + row];
+      bB = gptr(static_cast<const T *>(d.glambda))[(size_t)kg * n + row];
+    }
+  }
+
   T mM[PLDS ? 1 : CW], mS[SLDS ? 1 : CW];
   if constexpr (BC) {
     load_row<T, NS, true>(mM, static_cast<const T *>(d.Pinv), kg, row, it.N, it.act_u);
@@ -1455,24 +1479,18 @@ __device__ __forceinline__ void reg_item_pair(const KSys &d, uint32_t rank, T *l
   __syncthreads();
   it.base = (uint32_t)it.ctl[1];
   it.seq = 0;
-  {
-    const T *lam0 = static_cast<const T *>(d.lambda);
-    for (int i = opaque_tid(); i < (K + 2) * n; i += NT) {
-      const int jj = i / n, e = i - jj * n;
-      const int k = it.a - 1 + jj;
-      if (k >= 0 && k < it.N) it.xv[(jj + 1) * nP + e] = gptr(lam0)[(size_t)k * n + e];
-    }
+  if (lam_pos >= 0) it.xv[lam_pos] = lam_pre;
+  for (int i = opaque_tid() + NT; i < wl; i += NT) {
+    const int jj = i / n, e = i - jj * n;
+    const int k = it.a - 1 + jj;
+    if (k >= 0 && k < it.N) it.xv[(jj + 1) * nP + e] = gptr(lam0)[(size_t)k * n + e];
   }
   __syncthreads();
 
   RegOut fo{0, GBDPCG_MAX_ITER, 0.0, 0.0}, ao{0, GBDPCG_MAX_ITER, 0.0, 0.0};
   bool timed_out = false;
-  T xA = 0, xB = 0, bA = 0, bB = 0;
-  if (it.owner) {
-    xA = it.xv[own_pos];
-    bA = gptr(static_cast<const T *>(d.gamma))[(size_t)kg * n + row];
-    bB = gptr(static_cast<const T *>(d.glambda))[(size_t)kg * n + row];
-  }
+  T xA = 0, xB = 0;
+  if (it.owner) xA = it.xv[own_pos];
   // r = rhs - S x (B starts from x = 0: r = g_lambda)
   T rA_i;
   if constexpr (SLDS)
