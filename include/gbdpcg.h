@@ -76,9 +76,9 @@ extern "C" {
  * grid-wide reductions per iteration.  FUSED is the Chronopoulos-Gear form of the
  * same method, one reduction per iteration (identical in exact arithmetic).  FUSED
  * needs at least 2 knots per workgroup: a system that no layout gives that (N = 1,
- * or LDS-resident rows of odd N too large for a CU's LDS at 3 knots, as FP64 from
- * n = 25 on) runs the reference iteration instead, in single solves and batch
- * plans alike. */
+ * or LDS-resident rows too large for a CU's LDS at 2 or 3 knots per workgroup, as
+ * FP64 at odd N from n = 25 on and at any N > 2 for n = 32) runs the reference
+ * iteration instead, in single solves and batch plans alike. */
 #define GBDPCG_RECUR_REFERENCE 0
 #define GBDPCG_RECUR_FUSED 1
 
