@@ -1024,11 +1024,38 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     set_error("gbdpcg_tune_single: cannot tune inside graph capture");
     return GBDPCG_ERR_INVALID;
   }
-  const bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED && N >= 2;
+  bool fused = cfg->recurrence == GBDPCG_RECUR_FUSED && N >= 2;
   const size_t ts = tsize(dtype);
 
+  // candidates: the untuned default first, then every (kernel, threads, knots per
+  // workgroup, XCD placement) that fits
+  std::vector<Layout> cands;
+  auto add = [&](const Layout &L) {
+    for (const Layout &c : cands)
+      if (c.reg == L.reg && c.bc == L.bc && c.nt == L.nt && c.G == L.G && c.stride == L.stride) return;
+    cands.push_back(L);
+  };
+  {
+    Layout def;
+    rc = GBDPCG_ERR_UNSUPPORTED;
+    if ((cfg->kernel == GBDPCG_KERNEL_AUTO && dtype == GBDPCG_F64) || cfg->kernel == GBDPCG_KERNEL_ROWCAST)
+      rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused, true);
+    if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_REGISTER))
+      rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused);
+    if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_LDS))
+      rc = plan_single(dtype, n, N, fused, 0, 0, &def);
+    if (rc == GBDPCG_ERR_TOO_LARGE && fused && lds_fused_unfit(dtype, n, N)) {
+      // solves of this config run the reference iteration (solve_locked): tune
+      // those layouts, cached under that recurrence's key, which they look up
+      fused = false;
+      rc = plan_single(dtype, n, N, false, 0, 0, &def);
+    }
+    if (rc < 0) return rc;
+    add(def);
+  }
   // the cache holds the choice for automatic solves: read and written only when
-  // the config leaves the kernel and the threads per workgroup open
+  // the config leaves the kernel and the threads per workgroup open (looked up
+  // under the recurrence the default plan settled on)
   const bool open_cfg = cfg->kernel == GBDPCG_KERNEL_AUTO && cfg->threads == 0;
   // a choice already made for this shape (this process, or the cache file) is kept
   if (open_cfg) {
@@ -1049,26 +1076,6 @@ int gbdpcg_tune_single(int dtype, uint32_t n, uint32_t N, const void *d_S, const
     }
   }
 
-  // candidates: the untuned default first, then every (kernel, threads, knots per
-  // workgroup, XCD placement) that fits
-  std::vector<Layout> cands;
-  auto add = [&](const Layout &L) {
-    for (const Layout &c : cands)
-      if (c.reg == L.reg && c.bc == L.bc && c.nt == L.nt && c.G == L.G && c.stride == L.stride) return;
-    cands.push_back(L);
-  };
-  {
-    Layout def;
-    rc = GBDPCG_ERR_UNSUPPORTED;
-    if ((cfg->kernel == GBDPCG_KERNEL_AUTO && dtype == GBDPCG_F64) || cfg->kernel == GBDPCG_KERNEL_ROWCAST)
-      rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused, true);
-    if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_REGISTER))
-      rc = plan_single_reg(dtype, n, N, 0, 0, &def, 0, fused);
-    if (rc == GBDPCG_ERR_UNSUPPORTED && (cfg->kernel == GBDPCG_KERNEL_AUTO || cfg->kernel == GBDPCG_KERNEL_LDS))
-      rc = plan_single(dtype, n, N, fused, 0, 0, &def);
-    if (rc < 0) return rc;
-    add(def);
-  }
   const uint32_t Ks[] = {1, 2, 3, 4, 5, 6, 7, 8, 10, 12, 14, 16, 20, 24, 28, 32, 40, 48, 64};
   // kind 2: row-broadcast register layout, 1: register, 0: LDS
   for (int kind = 2; kind >= 0; --kind) {

@@ -47,6 +47,29 @@ def test_tune_single_then_solve(gpu, recurrence, N):
     assert (rep2["threads"], rep2["wgs"], rep2["kernel"]) == (rep["threads"], rep["wgs"], rep["kernel"])
 
 
+def test_tune_single_fused_request_without_fused_layout(gpu):
+    """FP64 n = 25, N = 5 has no LDS layout with 2 knots per workgroup: tuning a
+    fused request tunes the reference iteration's layouts, which the solve of the
+    same config then runs (at parity with the reference form), and a second call
+    finds that choice in the cache."""
+    import gbdpcg
+    n, N, dtype = 25, 5, np.float64
+    S, g = gbdpcg.lqr_generate(n, 12, N, 731, dtype=np.float64)
+    P = O.precond(S, n, N, 2)
+    kw = dict(tol=1e-9, max_iter=400, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+    dS, dP, dg = _t(S, dtype), _t(P, dtype), _t(g, dtype)
+    rep = gbdpcg.tune_single(dS, dg, n, N, dP, **kw)
+    assert rep["from_cache"] == 0 and rep["candidates"] >= 1 and rep["kernel"] == gbdpcg.KERNEL_LDS
+    lam = _t(np.zeros(n * N), dtype)
+    r = gbdpcg.solve(dS, dg, lam, n, N, dP, **kw)
+    assert r.status == gbdpcg.CONVERGED
+    lam_ref, it_ref, *_ = O.pcg(S, P, g, np.zeros(n * N), n, N, 1e-9, 400, O.EXIT_REL, np.float64, fused=False)
+    assert abs(r.iters - it_ref) <= 1
+    assert rel(lam.cpu().numpy(), O.dense_solve(S, g, n, N)) <= 1e-7
+    rep2 = gbdpcg.tune_single(dS, dg, n, N, dP, **kw)
+    assert rep2["from_cache"] == 1 and rep2["wgs"] == rep["wgs"]
+
+
 def batch_systems(specs, dtype, seed0=12000):
     import torch
     import gbdpcg
