@@ -176,6 +176,34 @@ def test_every_state_size(gpu, dtype, recurrence, kernel):
     assert fallbacks == 0 or (dtype == np.float64 and recurrence == 1)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_every_state_size_reference_rule(gpu, dtype, recurrence):
+    """The reference's exit rule |r'z| < tol (include/pcg.cuh:195) for every
+    stateSize 1..32 at N = 3 and 24 (automatic kernel), tol a fixed fraction of
+    the initial r'z = gamma' Pinv gamma (1e-9 in FP32, 1e-18 in FP64: about the
+    relative residuals 3e-5 and 1e-9 of the other sweeps); counts and lambda as
+    in test_every_state_size."""
+    import gbdpcg
+    lam_tol = 1e-4 if dtype == np.float32 else 1e-9
+    scale = 1e-9 if dtype == np.float32 else 1e-18
+    for n in range(1, 33):
+        for N in (3, 24):
+            S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, 3000 + 37 * n + N, dtype=np.float64)
+            P = O.precond(S, n, N, 2)
+            tol = scale * abs(float(g @ O.bdmv(P, g, n, N)))
+            if recurrence == 0:
+                ran = False
+            elif n in (2, 6, 12, 14, 28) and N >= 4:
+                ran = True
+            else:
+                G = N // 2
+                ran = gbdpcg.lds_bytes(dtype, n, -(-N // G), G) <= 160 * 1024
+            res, _, _ = check_against_oracle(S.astype(dtype), P.astype(dtype), g.astype(dtype), n, N, dtype, tol, 400,
+                                             0, recurrence, ran_fused=ran, lam_tol=lam_tol)
+            assert res.status == 0, (n, N, res.status)
+
+
 def test_edge_cases(gpu):
     import gbdpcg
     n, N, dtype = 4, 5, np.float64
