@@ -34,6 +34,23 @@ def test_host_pointer_equals_device_pointer(gpu, dtype, with_pinv):
     assert np.array_equal(S, S_before) and np.array_equal(g, g_before)  # inputs are read only
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_host_pointer_every_state_size(gpu, dtype):
+    """The host-pointer call bit-identical to the device-pointer call for every
+    stateSize 1..32 at N = 5 and 7 (staged arrays at offsets that are not
+    16-byte multiples for most sizes; the staging grows along the way)."""
+    import gbdpcg
+    kw = dict(KW, tol=3e-5 if dtype == np.float32 else 1e-9, max_iter=400)
+    for n in range(1, 33):
+        for N in (5, 7):
+            S, P, g = lqr_system(n, N, 5000 + 37 * n + N, dtype)
+            lam_d, res_d = gpu_solve(S, P, g, np.zeros(n * N, dtype), n, N, dtype, **kw)
+            lam_h = np.zeros(n * N, dtype)
+            it = gbdpcg.solve_host(S, g, lam_h, n, N, P, **kw)
+            assert res_d.status == gbdpcg.CONVERGED, (n, N)
+            assert it == res_d.iters and np.array_equal(lam_h, lam_d), (n, N)
+
+
 def test_host_pointer_reentrant_on_default_stream(gpu):
     """Three host threads, default stream, different (n, N) and data; one of them
     alternates sizes so the shared staging grows while the others run."""
