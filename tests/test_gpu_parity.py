@@ -204,6 +204,33 @@ def test_every_state_size_reference_rule(gpu, dtype, recurrence):
             assert res.status == 0, (n, N, res.status)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+def test_every_state_size_warm_start(gpu, dtype, recurrence):
+    """A non-zero initial guess (standard normal) for every stateSize 1..32 at
+    N = 3 and 24 (automatic kernel): the first residual gamma - S lambda0 uses
+    lambda0 one knot beyond each rank's range.  Counts and lambda as in
+    test_every_state_size."""
+    import gbdpcg
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    lam_tol = 1e-4 if dtype == np.float32 else 1e-9
+    for n in range(1, 33):
+        for N in (3, 24):
+            S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, 4000 + 37 * n + N, dtype=np.float64)
+            P = O.precond(S, n, N, 2)
+            lam0 = gbdpcg.randn(n * N, 4500 + n + N, dtype=np.float64)
+            if recurrence == 0:
+                ran = False
+            elif n in (2, 6, 12, 14, 28) and N >= 4:
+                ran = True
+            else:
+                G = N // 2
+                ran = gbdpcg.lds_bytes(dtype, n, -(-N // G), G) <= 160 * 1024
+            res, _, _ = check_against_oracle(S.astype(dtype), P.astype(dtype), g.astype(dtype), n, N, dtype, tol, 400,
+                                             1, recurrence, lam0=lam0, ran_fused=ran, lam_tol=lam_tol)
+            assert res.status == 0, (n, N, res.status)
+
+
 def test_edge_cases(gpu):
     import gbdpcg
     n, N, dtype = 4, 5, np.float64
