@@ -114,6 +114,27 @@ def test_batch_every_state_size(gpu, dtype, split):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("split", [False, True])
+def test_batch_per_system_wgs(gpu, dtype, split):
+    """Per-system workgroup counts (the system's `wgs`, the reference's
+    pcg_grid.x) are honoured in one plan (fused recurrence): one knot per
+    workgroup runs the reference iteration, the others the fused one; every
+    system against the oracle."""
+    import gbdpcg
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    req = [(7, 12, 1), (7, 12, 3), (7, 12, 12), (14, 32, 8), (14, 32, 4), (14, 40, 0), (6, 20, 10), (28, 16, 8)]
+    systems, host = make_batch([(n, N, tol) for n, N, _ in req], dtype, seed0=9600)
+    for d, (_, _, w) in zip(systems, req):
+        d["wgs"] = w
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED, split_launches=split)
+    got = b.wgs()
+    for (n, N, w), g in zip(req, got):
+        assert g == w or (w == 0 and 1 <= g <= N), (n, N, w, g)
+    b.run()
+    check(systems, host, b.results(), dtype, 1)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("split", [False, True])
 def test_batch_kinds_and_limits(gpu, dtype, split):
     """One plan (fused recurrence) mixing forward-only, forward + adjoint and
     adjoint-only systems (the last with a given lambda), per-system max_iter caps
