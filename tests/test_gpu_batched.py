@@ -112,6 +112,31 @@ def test_batch_every_state_size(gpu, dtype, split):
     check(systems, host, b.results(), dtype, 1, adjoint=True)
 
 
+def test_batch_results_device(gpu):
+    """gbdpcg_batch_results_device: the plan's device-resident result records
+    (what a captured graph's later kernels can read) hold the same bytes as the
+    host copy from gbdpcg_batch_results."""
+    import ctypes
+    import torch
+    import gbdpcg
+    specs = [(14, 32, 1e-5), (6, 20, 1e-5), (28, 16, 1e-5), (7, 9, 1e-5)]
+    systems, host = make_batch(specs, np.float32, adjoint=True, seed0=9650)
+    b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=gbdpcg.RECUR_FUSED)
+    b.run()
+    torch.cuda.synchronize()
+    ptr = gbdpcg.lib().gbdpcg_batch_results_device(b._h)
+    assert ptr
+    nbytes = len(specs) * ctypes.sizeof(gbdpcg.Result)
+    buf = (ctypes.c_uint8 * nbytes)()
+    # the HIP runtime the library itself links (torch may bundle another copy,
+    # which does not know this allocation)
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    assert hip.hipMemcpy(buf, ctypes.c_void_p(ptr), ctypes.c_size_t(nbytes), 2) == 0  # device to host
+    res = b.results()
+    assert bytes(buf) == b"".join(bytes(r) for r in res)
+    check(systems, host, res, np.float32, 1, adjoint=True)
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("split", [False, True])
 def test_batch_per_system_wgs(gpu, dtype, split):
