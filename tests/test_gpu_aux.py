@@ -332,6 +332,29 @@ def test_nonfinite_input(gpu, recurrence, kernel):
     assert r.status == gbdpcg.NONFINITE
 
 
+def test_check_occupancy(gpu):
+    """gbdpcg_check_occupancy (the reference's checkPcgOccupancy,
+    include/pcg.cuh:23-49, which exits instead of returning): OK with the
+    co-resident workgroup capacity (a multiple of the CU count) for a system
+    that fits, TOO_LARGE for one whose block row does not fit LDS even at one
+    knot per workgroup, INVALID for bad arguments."""
+    import ctypes
+    import torch
+    import gbdpcg
+    L = gbdpcg.lib()
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cap = ctypes.c_uint32(0)
+    assert L.gbdpcg_check_occupancy(gbdpcg.F32, 14, 128, ctypes.byref(cap)) == 0
+    assert cap.value >= cus and cap.value % cus == 0
+    for n, N in ((2, 3), (28, 512)):
+        assert L.gbdpcg_check_occupancy(gbdpcg.F64, n, N, ctypes.byref(cap)) == 0 and cap.value % cus == 0
+    # n = 64 FP64: one knot's S and Phi^-1 rows alone exceed the 160 KiB LDS
+    assert gbdpcg.lds_bytes(np.float64, 64, 1, 2) > 160 * 1024
+    assert L.gbdpcg_check_occupancy(gbdpcg.F64, 64, 8, ctypes.byref(cap)) == gbdpcg.ERR_TOO_LARGE
+    assert L.gbdpcg_check_occupancy(gbdpcg.F32, 0, 8, ctypes.byref(cap)) == gbdpcg.ERR_INVALID
+    assert L.gbdpcg_check_occupancy(7, 14, 8, ctypes.byref(cap)) == gbdpcg.ERR_INVALID
+
+
 def test_invalid_arguments(gpu):
     import gbdpcg
     n, N = 4, 8
