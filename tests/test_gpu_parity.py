@@ -253,6 +253,36 @@ def test_edge_cases(gpu):
     check_against_oracle(S2, P2, g2, n, 2, dtype, 1e-12, 50, 1, 1)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_breakdown(gpu, dtype, recurrence, kernel, mode):
+    """Exact breakdowns end with status BREAKDOWN, 0 iterations and lambda
+    untouched, as in the oracle: p'Sp = 0 on the first direction (S indefinite,
+    diag(1, -1) blocks, Pinv = I, gamma = (1, 1, 0, ...)), and r'z = 0 with r != 0
+    (Pinv indefinite, diag(1, -1) blocks, S = diag(2, 3) blocks).  The reference
+    divides by zero in the first case and reports convergence in the second."""
+    import gbdpcg
+    n, N = 2, 4
+    g = np.zeros(n * N)
+    g[0] = g[1] = 1.0
+    for sdiag, pdiag in (((1.0, -1.0), (1.0, 1.0)), ((2.0, 3.0), (1.0, -1.0))):
+        S = np.zeros((N, 3, n, n))
+        P = np.zeros((N, 3, n, n))
+        for k in range(N):
+            S[k, 1] = np.diag(sdiag)
+            P[k, 1] = np.diag(pdiag)
+        S, P = S.reshape(-1).astype(dtype), P.reshape(-1).astype(dtype)
+        gd = g.astype(dtype)
+        lam, res = gpu_solve(S, P, gd, np.zeros(n * N, dtype), n, N, dtype, tol=1e-6, max_iter=25, exit_mode=mode,
+                             recurrence=recurrence, kernel=kernel)
+        _, it, st, *_ = O.pcg(S, P, gd, np.zeros(n * N, dtype), n, N, 1e-6, 25, mode, dtype,
+                              fused=recurrence == 1)
+        assert st == O.BREAKDOWN and res.status == gbdpcg.BREAKDOWN, (sdiag, res.status)
+        assert res.iters == it == 0 and not lam.any()
+
+
 @pytest.mark.parametrize("n,N,dtype", [(14, 4096, np.float32), (28, 1024, np.float64)])
 def test_long_horizons(gpu, n, N, dtype):
     """Long horizons (hundreds of workgroups in one persistent launch) to the same
