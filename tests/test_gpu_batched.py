@@ -112,6 +112,46 @@ def test_batch_every_state_size(gpu, dtype, split):
     check(systems, host, b.results(), dtype, 1, adjoint=True)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("recurrence", [0, 1])
+@pytest.mark.parametrize("adjoint", [False, True])
+@pytest.mark.parametrize("split", [False, True])
+def test_batch_breakdown_does_not_stall(gpu, dtype, recurrence, adjoint, split):
+    """Systems that break down exactly (p'Sp = 0, or r'z = 0 with r != 0) in a
+    plan with ordinary ones: they end with BREAKDOWN after 0 iterations (both
+    sides when paired with an adjoint of the same right-hand side), and every
+    other system of the launch converges at parity with the oracle."""
+    import gbdpcg
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    ok_specs = [(14, 32, tol), (7, 9, tol), (2, 6, tol), (28, 8, tol)]
+    systems, host = make_batch(ok_specs, dtype, adjoint=adjoint, seed0=9750)
+    n, N = 2, 4
+    g = np.zeros(n * N)
+    g[0] = g[1] = 1.0
+    bad = []
+    for sdiag, pdiag in (((1.0, -1.0), (1.0, 1.0)), ((2.0, 3.0), (1.0, -1.0))):
+        S = np.zeros((N, 3, n, n))
+        P = np.zeros((N, 3, n, n))
+        for k in range(N):
+            S[k, 1] = np.diag(sdiag)
+            P[k, 1] = np.diag(pdiag)
+        d = dict(S=_t(S.reshape(-1), dtype), Pinv=_t(P.reshape(-1), dtype), gamma=_t(g, dtype),
+                 lam=_t(np.zeros(n * N), dtype), n=n, N=N, tol=tol, max_iter=173)
+        if adjoint:
+            d.update(glam=_t(g, dtype), ggamma=_t(np.zeros(n * N), dtype), gS=_t(np.zeros(3 * n * n * N), dtype),
+                     adjoint=True)
+        bad.append(d)
+    b = gbdpcg.Batch(bad[:1] + systems + bad[1:], exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence,
+                     split_launches=split)
+    b.run()
+    res = b.results()
+    for r, d in ((res[0], bad[0]), (res[-1], bad[1])):
+        assert r.status == gbdpcg.BREAKDOWN and r.iters == 0 and not d["lam"].cpu().numpy().any()
+        if adjoint:
+            assert r.adj_status == gbdpcg.BREAKDOWN and r.adj_iters == 0
+    check(systems, host, res[1:-1], dtype, recurrence, adjoint=adjoint)
+
+
 def test_batch_results_device(gpu):
     """gbdpcg_batch_results_device: the plan's device-resident result records
     (what a captured graph's later kernels can read) hold the same bytes as the
