@@ -117,10 +117,11 @@ def test_batch_every_state_size(gpu, dtype, split):
 @pytest.mark.parametrize("adjoint", [False, True])
 @pytest.mark.parametrize("split", [False, True])
 def test_batch_breakdown_does_not_stall(gpu, dtype, recurrence, adjoint, split):
-    """Systems that break down exactly (p'Sp = 0, or r'z = 0 with r != 0) in a
-    plan with ordinary ones: they end with BREAKDOWN after 0 iterations (both
-    sides when paired with an adjoint of the same right-hand side), and every
-    other system of the launch converges at parity with the oracle."""
+    """Systems that break down exactly (p'Sp = 0, or r'z = 0 with r != 0) or hold
+    a NaN, in a plan with ordinary ones: they end with BREAKDOWN / NONFINITE after
+    0 iterations (both sides when paired with an adjoint of the same right-hand
+    side), and every other system of the launch converges at parity with the
+    oracle."""
     import gbdpcg
     tol = 3e-5 if dtype == np.float32 else 1e-9
     ok_specs = [(14, 32, tol), (7, 9, tol), (2, 6, tol), (28, 8, tol)]
@@ -141,15 +142,26 @@ def test_batch_breakdown_does_not_stall(gpu, dtype, recurrence, adjoint, split):
             d.update(glam=_t(g, dtype), ggamma=_t(np.zeros(n * N), dtype), gS=_t(np.zeros(3 * n * n * N), dtype),
                      adjoint=True)
         bad.append(d)
+    # and a NaN in one right-hand side entry: NONFINITE
+    S, gq = gbdpcg.lqr_generate(n, 1, N, 9799, dtype=np.float64)
+    gq[3] = np.nan
+    d = dict(S=_t(S, dtype), Pinv=_t(O.precond(S, n, N, 2), dtype), gamma=_t(gq, dtype),
+             lam=_t(np.zeros(n * N), dtype), n=n, N=N, tol=tol, max_iter=173)
+    if adjoint:
+        d.update(glam=_t(gq, dtype), ggamma=_t(np.zeros(n * N), dtype), gS=_t(np.zeros(3 * n * n * N), dtype),
+                 adjoint=True)
+    bad.append(d)
     b = gbdpcg.Batch(bad[:1] + systems + bad[1:], exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=recurrence,
                      split_launches=split)
     b.run()
     res = b.results()
-    for r, d in ((res[0], bad[0]), (res[-1], bad[1])):
-        assert r.status == gbdpcg.BREAKDOWN and r.iters == 0 and not d["lam"].cpu().numpy().any()
+    k = len(systems)
+    for r, d, want in ((res[0], bad[0], gbdpcg.BREAKDOWN), (res[k + 1], bad[1], gbdpcg.BREAKDOWN),
+                       (res[k + 2], bad[2], gbdpcg.NONFINITE)):
+        assert r.status == want and r.iters == 0 and not d["lam"].cpu().numpy().any(), (r.status, want)
         if adjoint:
-            assert r.adj_status == gbdpcg.BREAKDOWN and r.adj_iters == 0
-    check(systems, host, res[1:-1], dtype, recurrence, adjoint=adjoint)
+            assert r.adj_status == want and r.adj_iters == 0
+    check(systems, host, res[1:k + 1], dtype, recurrence, adjoint=adjoint)
 
 
 def test_batch_results_device(gpu):
