@@ -177,6 +177,25 @@ def test_every_state_size(gpu, dtype, recurrence, kernel):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kernel", [2, 3])
+def test_register_kernels_every_horizon(gpu, dtype, kernel):
+    """The register layouts (column chunks: n in {2, 6, 12, 14, 28}; row
+    broadcast: n in {12, 14}) at every horizon N from 4 to 40 (fused recurrence,
+    the planner's rank split: uneven last ranks, one to several ranks), against
+    the oracle."""
+    import gbdpcg
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    sizes = (2, 6, 12, 14, 28) if kernel == 2 else (12, 14)
+    for n in sizes:
+        for N in range(4, 41):
+            S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, 7000 + 41 * n + N, dtype=np.float64)
+            P = O.precond(S, n, N, 2)
+            res, _, _ = check_against_oracle(S.astype(dtype), P.astype(dtype), g.astype(dtype), n, N, dtype, tol, 400,
+                                             1, 1, kernel=kernel, lam_tol=1e-4 if dtype == np.float32 else 1e-9)
+            assert res.status == 0, (n, N, res.status)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("recurrence", [0, 1])
 def test_every_state_size_reference_rule(gpu, dtype, recurrence):
     """The reference's exit rule |r'z| < tol (include/pcg.cuh:195) for every
