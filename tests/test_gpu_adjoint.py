@@ -141,6 +141,35 @@ def test_adjoint_every_state_size(gpu, dtype, recurrence):
 
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("kernel", [2, 3])
+def test_paired_every_horizon(gpu, dtype, kernel):
+    """Forward + adjoint in one loop on the register layouts (column chunks: n in
+    {2, 6, 12, 14}; row broadcast: n in {12, 14}) at every horizon N from 4 to
+    40, against the oracle: both counts within one of both forms' same-precision
+    counts, lambda and g_gamma at the FP64 iterates of those counts, g_S by the
+    formula."""
+    tol = 3e-5 if dtype == np.float32 else 1e-9
+    bar = 1e-4 if dtype == np.float32 else 1e-9
+    sizes = (2, 6, 12, 14) if kernel == 2 else (12, 14)
+    for n in sizes:
+        for N in range(4, 41):
+            S, P, g, glam = system(n, N, 7500 + 41 * n + N, dtype)
+            lam, gg, gS, res = run_adjoint(S, P, g, glam, n, N, dtype, tol=tol, max_iter=400, exit_mode=1,
+                                           recurrence=1, kernel=kernel)
+            assert res.status == 0 and res.adj_status == 0, (n, N, res.status, res.adj_status)
+            for rhs, its, x in ((g, res.iters, lam), (glam, res.adj_iters, gg)):
+                for f in (True, False):
+                    _, it, *_ = O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), np.zeros(n * N, dtype), n,
+                                      N, tol, 400, 1, dtype, fused=f)
+                    assert abs(its - it) <= 1, (n, N, its, it, f)
+                ref, *_ = O.pcg(S, P, rhs, np.zeros(n * N), n, N, 0.0, its, 1, np.float64, fused=True)
+                assert rel(x, ref) <= bar, (n, N, rel(x, ref))
+            expect = O.dense_to_cbtd(-0.5 * (np.outer(gg, lam) + np.outer(lam, gg)), n, N) * O.pattern_mask(n, N)
+            assert np.allclose(gS, expect, rtol=1e-5 if dtype == np.float32 else 1e-12,
+                               atol=1e-6 * np.abs(expect).max()), (n, N)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("kernel", [1, 2, 3])
 def test_sides_stop_independently(gpu, dtype, kernel):
     """Forward + adjoint, fused recurrence (the register kernels run both solves in
