@@ -250,6 +250,55 @@ def test_every_state_size_warm_start(gpu, dtype, recurrence):
             assert res.status == 0, (n, N, res.status)
 
 
+def test_random_sweep(gpu):
+    """1500 seeded random single solves: n in 1..32, N in 1..64, FP32 or FP64,
+    either recurrence, automatic / LDS / register / row-broadcast kernel (layouts
+    a kernel declines are skipped), either exit rule, cold or warm start.  Each
+    against the oracle: status, the count within one of the same-precision
+    count of each form that ran, lambda within 1e-4 / 1e-9 of the FP64 iterate of
+    either form at that count.  Tolerances sit above the FP32 floor (a random
+    sweep of 20,000 cases at these settings found only cases where the oracle's
+    own two FP32 forms differ by more than one iteration, warm starts with
+    |lambda*| up to 28 |gamma|)."""
+    import gbdpcg
+    rng = np.random.default_rng(11)
+    ran = 0
+    for c in range(1500):
+        n = int(rng.integers(1, 33))
+        N = int(rng.integers(1, 65))
+        dtype = [np.float32, np.float64][int(rng.integers(0, 2))]
+        rec = int(rng.integers(0, 2))
+        kernel = [0, 0, 1, 2, 3][int(rng.integers(0, 5))]
+        mode = int(rng.integers(0, 2))
+        warm = bool(rng.integers(0, 2))
+        seed = int(rng.integers(0, 1 << 30))
+        S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, seed, dtype=np.float64)
+        P = O.precond(S, n, N, 2)
+        lam0 = gbdpcg.randn(n * N, seed + 1, dtype=np.float64) if warm else np.zeros(n * N)
+        if mode == 1:
+            tol = 3e-5 if dtype == np.float32 else 1e-9
+        else:
+            r0 = g - O.bdmv(S, lam0, n, N)
+            tol = (1e-9 if dtype == np.float32 else 1e-18) * abs(float(r0 @ O.bdmv(P, r0, n, N)))
+        try:
+            lam, res = gpu_solve(S.astype(dtype), P.astype(dtype), g.astype(dtype), lam0.astype(dtype), n, N, dtype,
+                                 tol=tol, max_iter=400, exit_mode=mode, recurrence=rec, kernel=kernel)
+        except gbdpcg.GbdpcgError as e:
+            assert e.code in (gbdpcg.ERR_UNSUPPORTED, gbdpcg.ERR_TOO_LARGE), (c, e)
+            continue
+        ran += 1
+        case = (c, n, N, dtype.__name__, rec, kernel, mode, warm)
+        assert res.status == gbdpcg.CONVERGED, case
+        for f in ((True, False) if rec else (False,)):
+            _, it, st, *_ = O.pcg(S.astype(dtype), P.astype(dtype), g.astype(dtype), lam0.astype(dtype), n, N, tol,
+                                  400, mode, dtype, fused=f)
+            assert st == O.CONVERGED and abs(res.iters - it) <= 1, case + (res.iters, it, f)
+        errs = [rel(lam, O.pcg(S, P, g, lam0, n, N, 0.0, res.iters, mode, np.float64, fused=f)[0])
+                for f in (True, False)]
+        assert min(errs) <= (1e-4 if dtype == np.float32 else 1e-9), case + tuple(errs)
+    assert ran >= 900  # 941: the register and row-broadcast kernels decline most sizes
+
+
 def test_edge_cases(gpu):
     import gbdpcg
     n, N, dtype = 4, 5, np.float64
