@@ -197,6 +197,75 @@ def test_n28_paired_every_horizon(gpu, dtype):
                            atol=1e-6 * np.abs(expect).max()), N
 
 
+def test_random_plans(gpu):
+    """60 seeded random plans (1-40 systems each: n in {1, 2, 3, 6, 7, 12, 14, 20,
+    28}, N in 1..96, forward-only / forward + adjoint / adjoint-only, some capped
+    at 4 iterations; FP32 or FP64, either recurrence, one launch or split), every
+    side of every system against the oracle: a binding cap ends in MAX_ITER after
+    exactly the cap; otherwise CONVERGED with the count within one of each form's
+    same-precision count, except where those two counts differ by more than one
+    themselves (an FP32 fused solve at the precision floor), which is held to
+    convergence only; the result within 1e-4 / 1e-9 of the FP64 iterate of either
+    form at its count; g_S by the formula."""
+    import gbdpcg
+    rng = np.random.default_rng(5)
+    for pi in range(60):
+        dtype = [np.float32, np.float64][int(rng.integers(0, 2))]
+        rec = int(rng.integers(0, 2))
+        split = bool(rng.integers(0, 2))
+        count = int(rng.integers(1, 41))
+        tol = 3e-5 if dtype == np.float32 else 1e-9
+        systems, meta = [], []
+        for i in range(count):
+            n = int(rng.choice([1, 2, 3, 6, 7, 12, 14, 20, 28]))
+            N = int(rng.integers(1, 97))
+            kind = ["fwd", "both", "adj"][int(rng.integers(0, 3))]
+            mx = int(rng.choice([173, 173, 173, 4]))
+            seed = int(rng.integers(0, 1 << 30))
+            S, g = gbdpcg.lqr_generate(n, max(n // 2, 1), N, seed, dtype=np.float64)
+            P = O.precond(S, n, N, 2)
+            glam = gbdpcg.randn(n * N, seed + 7, dtype=np.float64)
+            lam_in = O.dense_solve(S, g, n, N) if kind == "adj" else np.zeros(n * N)
+            d = dict(S=_t(S.astype(dtype), dtype), Pinv=_t(P.astype(dtype), dtype), gamma=_t(g.astype(dtype), dtype),
+                     lam=_t(lam_in.astype(dtype), dtype), n=n, N=N, tol=tol, max_iter=mx, forward=kind != "adj",
+                     adjoint=kind != "fwd")
+            if kind != "fwd":
+                d.update(glam=_t(glam.astype(dtype), dtype), ggamma=_t(np.zeros(n * N), dtype),
+                         gS=_t(np.zeros(3 * n * n * N), dtype))
+            systems.append(d)
+            meta.append((n, N, kind, mx, S, P, g, glam))
+        b = gbdpcg.Batch(systems, exit_mode=gbdpcg.EXIT_REL_RESID, recurrence=rec, split_launches=split)
+        b.run()
+
+        def side(case, S, P, rhs, n, N, mx, its, st, x):
+            z = np.zeros(n * N, dtype)
+            forms = (True, False) if rec else (False,)
+            counts = [O.pcg(S.astype(dtype), P.astype(dtype), rhs.astype(dtype), z, n, N, tol, mx, O.EXIT_REL, dtype,
+                            fused=f)[1:3] for f in forms]
+            if counts[0][1] == O.MAX_ITER:
+                assert st == gbdpcg.MAX_ITER and its == mx, case + (its, st, counts)
+            else:
+                assert st == gbdpcg.CONVERGED, case + (its, st, counts)
+                floor = len(counts) == 2 and abs(counts[0][0] - counts[1][0]) > 1
+                assert floor or all(abs(its - c) <= 1 for c, _ in counts), case + (its, counts)
+            errs = [rel(x, O.pcg(S, P, rhs, np.zeros(n * N), n, N, 0.0, its, O.EXIT_REL, np.float64, fused=f)[0])
+                    for f in (True, False)]
+            assert min(errs) <= (1e-4 if dtype == np.float32 else 1e-9), case + tuple(errs)
+
+        for i, (d, (n, N, kind, mx, S, P, g, glam), r) in enumerate(zip(systems, meta, b.results())):
+            case = (pi, i, dtype.__name__, rec, split, n, N, kind, mx)
+            lam = d["lam"].cpu().numpy().astype(np.float64)
+            if kind != "adj":
+                side(case, S, P, g, n, N, mx, r.iters, r.status, lam)
+            if kind != "fwd":
+                gg = d["ggamma"].cpu().numpy().astype(np.float64)
+                side(case + ("adjoint",), S, P, glam, n, N, mx, r.adj_iters, r.adj_status, gg)
+                expect = O.dense_to_cbtd(-0.5 * (np.outer(gg, lam) + np.outer(lam, gg)), n, N) * O.pattern_mask(n, N)
+                assert np.allclose(d["gS"].cpu().numpy(), expect, rtol=1e-5 if dtype == np.float32 else 1e-12,
+                                   atol=1e-6 * max(np.abs(expect).max(), 1e-300)), case
+        b.close()
+
+
 def test_batch_results_device(gpu):
     """gbdpcg_batch_results_device: the plan's device-resident result records
     (what a captured graph's later kernels can read) hold the same bytes as the
